@@ -1,0 +1,347 @@
+/*
+ * nbl_hip.h -- C ABI of the MI355X electron-transport library (libnbl_hip.so).
+ *
+ * The reference (Nebula 1.0.2) exposes this hot path as compile-time C++
+ * templates, not as an FFI.  Each entry point below replaces one member of
+ * that template API; the replaced reference interface is cited per function
+ * (paths relative to the reference's source/ directory).
+ *
+ * Conventions
+ *   - Every function returns an int status: NBL_OK (0) or a negative NBL_ERR_*.
+ *     nbl_last_error() returns a thread-local message for the last failure.
+ *   - No C++ exception crosses this boundary.
+ *   - Pointers to particles, triangles, tables and results are HOST pointers
+ *     owned by the caller for the duration of the call.  The library owns all
+ *     device memory it allocates and its own HIP streams.
+ *   - Handles are not thread-safe; use one driver per host thread and device
+ *     (as gpu_main.cu:77-82 does).
+ *   - All arithmetic is IEEE float32 (real=float, config/config.h:13).
+ */
+#ifndef NBL_HIP_H
+#define NBL_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define NBL_OK             0
+#define NBL_ERR_INVALID   (-1) /* bad argument or handle */
+#define NBL_ERR_ALLOC     (-2) /* device/host allocation failed (bad_alloc) */
+#define NBL_ERR_HIP       (-3) /* HIP runtime error */
+#define NBL_ERR_RUNTIME   (-4) /* runtime_error: malformed input, limits */
+#define NBL_ERR_NODEVICE  (-5) /* no HIP device visible */
+
+/* Triangle id meaning "no triangle" (the reference's nullptr triangle*). */
+#define NBL_NO_TRIANGLE 0xFFFFFFFFu
+
+/* Special (non-physical) material indices: core/cpu_material_manager.h:82-91 */
+#define NBL_MAT_NOP            (-128)
+#define NBL_MAT_TERMINATOR     (-127)
+#define NBL_MAT_DETECTOR       (-126)
+#define NBL_MAT_DETECTOR_LT50  (-125)
+#define NBL_MAT_DETECTOR_GE50  (-124)
+#define NBL_MAT_VACUUM         (-123)
+#define NBL_MAT_MIRROR         (-122)
+
+/* ---- plain data types, layout-identical to the reference --------------- */
+typedef struct { float x, y, z; } nbl_vec3;                          /* vec3, config/data_types.h:38-40 */
+typedef struct { nbl_vec3 pos, dir; float kin_energy; } nbl_particle; /* particle, core/particle.h:5-10 (28 B) */
+typedef struct {                                                     /* triangle, core/triangle.h:8-77 (44 B) */
+    int32_t material_in;  /* material on the side the normal points away from */
+    int32_t material_out; /* material on the side the normal points towards   */
+    nbl_vec3 r0, e1, e2;  /* vertex 0 and the edges r1-r0, r2-r0             */
+} nbl_triangle;
+
+/* 1D table: data[i] at x = x_min + i*(x_max-x_min)/(n-1)  (common/util/table_1D.h:46-48) */
+typedef struct {
+    int32_t n;
+    float x_min, x_max;
+    const float* data;
+} nbl_table1d_desc;
+
+/* 2D table: value(i,j) = data[i*height + j]; i along x (width), j along y
+ * (height).  Same storage order as table_2D<real,false> (table_2D.inl:151-157). */
+typedef struct {
+    int32_t width, height;
+    float x_min, x_max, y_min, y_max;
+    const float* data;
+} nbl_table2d_desc;
+
+/* A material with Kieft-Bosch elastic + inelastic physics, tables already
+ * prepared as the reference's create() functions leave them
+ * (physics/kieft/elastic.h:120-166, physics/kieft/inelastic.h:212-327). */
+typedef struct {
+    float barrier;      /* E_F + work function, eV (core/material.h:52-57)            */
+    float fermi;        /* eV (inelastic.h:220)                                       */
+    float band_gap;     /* eV; <0 for metals (inelastic.h:221)                        */
+    float phonon_loss;  /* eV (elastic.h:160)                                         */
+    float recoil_const; /* 2*(2 m_e)/M_atom (elastic.h:161-163)                       */
+    nbl_table1d_desc elastic_log_imfp;       /* log(imfp/nm^-1) vs log(K/eV)          */
+    nbl_table2d_desc elastic_costheta_icdf;  /* cos(theta) vs (log K, P), clamped     */
+    nbl_table1d_desc inelastic_log_imfp;     /* log(imfp/nm^-1) vs log(K/eV)          */
+    nbl_table2d_desc inelastic_log_w0_icdf;  /* log(omega'/eV) vs (log K, P)          */
+    nbl_table2d_desc ionisation;             /* binding/eV vs (log omega', P), rounddown */
+} nbl_material_desc;
+
+/* Raw cross-section data as a cross-section tool writes it, before the
+ * reference's create() post-processing.  All energy grids are log-spaced
+ * from k_min to k_max (eV); all probability grids linear from p_min to p_max. */
+typedef struct {
+    float barrier, fermi, band_gap, phonon_loss;
+    double effective_a_grams;       /* mass of the effective atom, g (elastic.h:161-163) */
+    int32_t n_k;                    /* energy points, shared by every table             */
+    double k_min, k_max;            /* eV                                               */
+    int32_t n_p;                    /* probability points, shared by every 2D table     */
+    double p_min, p_max;
+    const float* elastic_imfp;      /* [n_k]        nm^-1                               */
+    const float* elastic_costheta;  /* [n_k][n_p]   cos(theta) ICDF                     */
+    const float* inelastic_imfp;    /* [n_k]        nm^-1                               */
+    const float* inelastic_w0;      /* [n_k][n_p]   omega' ICDF, eV                     */
+    const float* binding_icdf;      /* [n_k][n_p]   binding energy ICDF, eV (K axis = omega') */
+    int32_t n_outer_shells;
+    const double* outer_shells;     /* eV, any order                                    */
+} nbl_kieft_raw;
+
+/* ---- library ------------------------------------------------------------ */
+const char* nbl_last_error(void);
+int nbl_version(int* major, int* minor, int* patch);
+/* Name of device 0 and its compute-unit count. */
+int nbl_device_info(char* name, size_t name_len, int* compute_units);
+
+/* ---- geometry: replaces geometry::octree<true> (geometry/octree.h:31-111) --- */
+typedef struct nbl_geometry_s* nbl_geometry;
+
+typedef struct {
+    nbl_vec3 aabb_min, aabb_max;  /* simulation domain (octree.h:93-97)              */
+    float max_extent;             /* 2*|halfsize| (octree.inl:300)                    */
+    uint32_t n_triangles_in;      /* triangles given to create()                       */
+    uint32_t n_triangles_unique;  /* triangles kept after duplicate removal            */
+    uint32_t n_nodes;             /* internal octree nodes (root included)             */
+    uint32_t n_leaves;
+    uint32_t n_leaf_refs;         /* leaf->triangle references                         */
+    uint32_t max_depth;
+    uint64_t linearized_size;     /* size of the reference-format octree_data array    */
+    uint64_t device_bytes;        /* device memory held by the geometry                */
+} nbl_geometry_info;
+
+/* octree<true>::create(std::vector<triangle>) (octree.h:42-49, octree.inl:8-21):
+ * builds the octree with the reference's subdivision rules on the host, then
+ * uploads it.  Triangle ids returned by propagate are indices into tris. */
+int nbl_geometry_create(const nbl_triangle* tris, uint32_t n, nbl_geometry* out);
+/* octree<true>::destroy (octree.h:54) */
+int nbl_geometry_destroy(nbl_geometry g);
+int nbl_geometry_get_info(nbl_geometry g, nbl_geometry_info* info);
+/* The octree in the reference's linearized form (octree_builder.h:209-248):
+ * octree_data (linearized_size ints) and, for each position of triangle_data,
+ * the index of that triangle in the create() input.  Either pointer may be NULL. */
+int nbl_geometry_export_linearized(nbl_geometry g, int32_t* octree_data, uint32_t* triangle_order);
+/* linearized_octree(octree_root(triangles)) (octree_builder.cpp:171-352) on
+ * the host only, no device needed: the reference-format octree_data
+ * (*octree_size ints; pass octree_data = NULL to query the size), the input
+ * index of each triangle_data position (*n_unique entries) and the root box.
+ * Any output pointer may be NULL. */
+int nbl_octree_linearize(const nbl_triangle* tris, uint32_t n, int32_t* octree_data,
+    uint64_t* octree_size, uint32_t* triangle_order, uint32_t* n_unique,
+    float* center3, float* halfsize3);
+/* octree::in_domain (octree.inl:24-30), batched on the device. */
+int nbl_geometry_in_domain_batch(nbl_geometry g, const nbl_vec3* pos, uint32_t n, uint8_t* out);
+/* octree::propagate (octree.inl:32-192), batched on the device.  One ray per
+ * element; ignore_tri = NBL_NO_TRIANGLE for none.  out_tri = NBL_NO_TRIANGLE
+ * when no triangle is hit; out_dist as the reference's isect_distance. */
+int nbl_geometry_propagate_batch(nbl_geometry g,
+    const nbl_vec3* pos, const nbl_vec3* dir, const float* dist,
+    const uint32_t* ignore_tri, const int32_t* ignore_mat, uint32_t n,
+    float* out_dist, uint32_t* out_tri);
+/* Same, with pointers to DEVICE memory and no host staging (the timed form). */
+int nbl_geometry_propagate_batch_device(nbl_geometry g,
+    const nbl_vec3* pos, const nbl_vec3* dir, const float* dist,
+    const uint32_t* ignore_tri, const int32_t* ignore_mat, uint32_t n,
+    float* out_dist, uint32_t* out_tri);
+
+/* ---- materials: replaces gpu_material_manager (core/gpu_material_manager.h:19-79) */
+typedef struct nbl_materials_s* nbl_materials;
+
+/* kieft_elastic::create + kieft_inelastic::create table post-processing
+ * (elastic.h:120-166, inelastic.h:212-327) on the host.  The five output
+ * arrays must hold n_k, n_k*n_p, n_k, n_k*n_p and n_k*n_p floats; desc is
+ * filled to point at them. */
+int nbl_kieft_prepare(const nbl_kieft_raw* raw,
+    float* el_log_imfp, float* el_icdf, float* in_log_imfp,
+    float* in_log_w0_icdf, float* ionisation, nbl_material_desc* desc);
+/* gpu_material_manager::create(cpu_material_manager const&)
+ * (gpu_material_manager.inl:5-26): uploads n materials. */
+int nbl_materials_create(const nbl_material_desc* mats, int32_t n, nbl_materials* out);
+int nbl_materials_destroy(nbl_materials m);
+/* Maximal energy accepted by all materials (cpu_material_manager::get_max_energy,
+ * scatter_list.h:64-69, elastic.h:46-49). */
+int nbl_materials_get_max_energy(nbl_materials m, float* out);
+
+/* Table selectors for the lookup entry points. */
+#define NBL_TAB_ELASTIC_IMFP      0 /* 1D */
+#define NBL_TAB_INELASTIC_IMFP    1 /* 1D */
+#define NBL_TAB_ELASTIC_ICDF      2 /* 2D */
+#define NBL_TAB_INELASTIC_W0_ICDF 3 /* 2D */
+#define NBL_TAB_IONISATION        4 /* 2D */
+/* table_1D::get (table_1D.inl:65-81), batched on the device. */
+int nbl_table1d_get_batch(nbl_materials m, int32_t material, int32_t table,
+    const float* x, uint32_t n, float* out);
+/* table_2D::get (table_2D.inl:73-94) or, with rounddown != 0,
+ * table_2D::get_rounddown (table_2D.inl:96-104), batched on the device. */
+int nbl_table2d_get_batch(nbl_materials m, int32_t material, int32_t table,
+    int32_t rounddown, const float* x, const float* y, uint32_t n, float* out);
+
+/* ---- single-event physics with injected uniforms (unit-parity entry points) */
+/* Event kinds. */
+#define NBL_EVENT_SAMPLE_PATH 0 /* material::sample_path (scatter_list.h:106-112)        */
+#define NBL_EVENT_ELASTIC     1 /* kieft_elastic::execute (elastic.h:67-115)              */
+#define NBL_EVENT_INELASTIC   2 /* kieft_inelastic::execute (inelastic.h:68-206)          */
+#define NBL_EVENT_INTERSECT   3 /* boundary_intersect::execute (boundary_intersect.h:39-156) */
+/* Outcome codes written to out_status. */
+#define NBL_OUT_ALIVE       0
+#define NBL_OUT_DETECTED    1
+#define NBL_OUT_TERMINATED  2
+#define NBL_NUM_UNIFORMS    8 /* uniforms reserved per particle; draws are taken in order */
+/* Runs one event for each of n particles on the device.  For particle i the
+ * j-th uniform draw returns uniforms[i*NBL_NUM_UNIFORMS + j].
+ *   material[i]      current material index; last_tri[i] the intersected triangle
+ *                    (NBL_EVENT_INTERSECT only; g may be NULL otherwise)
+ *   io               particles, updated in place
+ *   out_material     new material index (changes only on transmission)
+ *   out_status       NBL_OUT_*
+ *   out_secondary    secondary electron (NBL_EVENT_INELASTIC); out_has_secondary 0/1
+ *   out_scalar       NBL_EVENT_SAMPLE_PATH: free path; out_type: event type 0/1/2
+ *                    (scatter_event, core/events.h:9-18) */
+int nbl_physics_batch(nbl_materials m, nbl_geometry g, int32_t event,
+    const int32_t* material, const uint32_t* last_tri, const float* uniforms,
+    uint32_t n, nbl_particle* io, int32_t* out_material, uint8_t* out_status,
+    nbl_particle* out_secondary, uint8_t* out_has_secondary,
+    float* out_scalar, uint8_t* out_type);
+
+/* ---- driver: replaces drivers::gpu_driver (drivers/gpu/gpu_driver.h:25-229) - */
+typedef struct nbl_driver_s* nbl_driver;
+
+typedef struct {
+    uint32_t threads_per_block; /* 0 = library default (256)                      */
+    uint32_t reserved[7];
+} nbl_driver_opts;
+
+/* gpu_driver::gpu_driver (gpu_driver.inl:51-89).  seed seeds the per-slot
+ * XORWOW streams (Marsaglia 2003; project seeding, not cuRAND's). */
+int nbl_driver_create(uint32_t capacity, nbl_geometry g, nbl_materials m,
+    float min_energy, float max_energy, uint64_t seed,
+    const nbl_driver_opts* opts, nbl_driver* out);
+/* gpu_driver::~gpu_driver (gpu_driver.inl:95-117) */
+int nbl_driver_destroy(nbl_driver d);
+/* gpu_driver::push (gpu_driver.inl:123-130, gpu_particle_manager.inl:105-164) */
+int nbl_driver_push(nbl_driver d, const nbl_particle* particles,
+    const uint32_t* tags, uint32_t n, uint32_t* pushed);
+/* gpu_driver::do_iteration (gpu_driver.inl:312-317): init, sort, events. */
+int nbl_driver_do_iteration(nbl_driver d);
+/* gpu_driver::get_running_count / get_detected_count (gpu_driver.inl:143-160) */
+int nbl_driver_get_running_count(nbl_driver d, uint32_t* out);
+int nbl_driver_get_detected_count(nbl_driver d, uint32_t* out);
+/* Detected-particle callback: (particle, tag of the primary, user). */
+typedef void (*nbl_detect_fn)(const nbl_particle* p, uint32_t tag, void* user);
+/* gpu_driver::flush_detected (gpu_driver.inl:132-141, gpu_particle_manager.inl:206-235) */
+int nbl_driver_flush_detected(nbl_driver d, nbl_detect_fn fn, void* user);
+/* gpu_driver::allocate_input_buffers (gpu_driver.inl:163-212) */
+int nbl_driver_allocate_input_buffers(nbl_driver d, uint32_t n);
+/* work_pool::get_work(bool*, particle*, uint32_t*, N) (common/work_pool.h:69-88):
+ * fill the free entries (data[i]==0) of the n-long input buffer, set data[i]=1. */
+typedef void (*nbl_pool_fill_fn)(uint8_t* data, nbl_particle* particles,
+    uint32_t* tags, uint32_t n, void* user);
+/* gpu_driver::push_to_buffer (gpu_driver.inl:214-242) */
+int nbl_driver_push_to_buffer(nbl_driver d, nbl_pool_fill_fn fill, void* user);
+/* gpu_driver::push_to_simulation (gpu_driver.inl:244-253) */
+int nbl_driver_push_to_simulation(nbl_driver d);
+/* gpu_driver::buffer_detected (gpu_driver.inl:255-269) */
+int nbl_driver_buffer_detected(nbl_driver d);
+/* gpu_driver::flush_buffered (gpu_driver.inl:271-306); *running = active count */
+int nbl_driver_flush_buffered(nbl_driver d, nbl_detect_fn fn, void* user, uint32_t* running);
+
+/* Slot-state access for parity tests and tools (no reference counterpart:
+ * the reference reads these arrays through cuda_mem_scope round trips,
+ * common/cuda/cuda_mem_scope.h:21-38).  Arrays are capacity long; any
+ * pointer may be NULL.  rng holds 6 uint32 per slot. */
+typedef struct {
+    uint8_t* status; uint32_t* particle_idx; nbl_particle* particles;
+    uint32_t* tags; int32_t* material; uint32_t* last_tri; uint32_t* rng;
+} nbl_slot_view;
+int nbl_driver_read_slots(nbl_driver d, const nbl_slot_view* v);
+int nbl_driver_write_slots(nbl_driver d, const nbl_slot_view* v);
+/* Status codes of a slot (gpu_particle_manager.h:93-103). */
+#define NBL_ST_PENDING         0x0
+#define NBL_ST_INELASTIC_EVENT 0x4
+#define NBL_ST_ELASTIC_EVENT   0x1
+#define NBL_ST_INTERSECT_EVENT 0x2
+#define NBL_ST_NO_EVENT        0x6
+#define NBL_ST_DETECTED        0xA
+#define NBL_ST_NEW_SECONDARY   0xE
+#define NBL_ST_TERMINATED      0x3
+
+/* Stable sort of n one-byte keys on bits [0,2), payload = index: the
+ * in-house replacement of cub::DeviceRadixSort::SortPairs(..., 0, 2)
+ * (gpu_particle_manager.inl:237-246).  Host arrays. */
+int nbl_sort_status_batch(const uint8_t* keys, uint32_t n, uint32_t* out_index);
+
+/* ---- XORWOW streams (replaces cuRAND, common/util/random.inl:35-62) ------- */
+/* Seed n states for sequences first_sequence .. first_sequence+n-1; 6 words each. */
+int nbl_rng_seed(uint64_t seed, uint64_t first_sequence, uint32_t n, uint32_t* states);
+/* Draw `draws` uniforms in (0,1] from each of n states, on the device. */
+int nbl_rng_uniform_batch(const uint32_t* states, uint32_t n, uint32_t draws, float* out);
+
+/* ---- persistent batched transport (no reference counterpart: replaces the
+ * prescan + frame loop of gpu_main.cu:108-213 with one launch) ------------ */
+/* A scan pixel: n primaries entering at (x,y,z) + beam spread, direction dir. */
+typedef struct {
+    float x, y, z;           /* beam centre, nm                                     */
+    float energy;            /* eV                                                  */
+    float sigma;             /* Gaussian beam sigma, nm (profile 0)                 */
+    float half_x, half_y;    /* half widths of a uniform rectangle, nm (profile 1)  */
+    uint32_t profile;        /* 0 = Gaussian, 1 = uniform rectangle                 */
+    uint32_t n_primaries;
+    uint32_t reserved;
+} nbl_pixel;
+typedef struct {
+    const nbl_pixel* pixels;
+    uint32_t n_pixels;
+    nbl_vec3 dir;            /* common beam direction (normalised by the kernel)   */
+    uint64_t seed;           /* primary-sampling seed                              */
+} nbl_pixel_batch;
+
+#define NBL_HIST_BINS 1024
+typedef struct {
+    uint64_t primaries;      /* primaries whose cascades ended                    */
+    uint64_t detected_se;    /* detected with E < 50 eV                            */
+    uint64_t detected_bse;   /* detected with E >= 50 eV                           */
+    uint64_t events;         /* init + event steps (one per active particle step)  */
+    uint64_t lookups_1d, lookups_2d, rounddowns;
+    uint64_t node_steps, tri_tests, boundary_events, secondaries, pending_retries;
+    double   kernel_ms;      /* device time of the transport launch(es)            */
+    uint32_t* pixel_se;      /* optional [n_pixels] out (may be NULL)              */
+    uint32_t* pixel_bse;     /* optional [n_pixels] out (may be NULL)              */
+    uint32_t* hist;          /* optional [NBL_HIST_BINS] detected-energy histogram, 1-eV bins */
+} nbl_tallies;
+
+typedef struct {
+    uint32_t block_size;       /* threads per workgroup (64..1024, multiple of 64); 0 = default */
+    uint32_t blocks_per_cu;    /* resident workgroups per CU; 0 = default          */
+    uint32_t stack_depth;      /* per-wavefront secondary stack entries; 0 = default */
+    uint32_t refill_threshold; /* refill a wavefront when >= this many lanes idle; 0 = default (1) */
+    uint32_t reserved[4];
+} nbl_persistent_opts;
+
+/* Runs every primary of the batch to the end of its cascade in one
+ * persistent launch; per-pixel SE/BSE tallies and counters go to *out. */
+int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch,
+    const nbl_persistent_opts* opts, nbl_tallies* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBL_HIP_H */

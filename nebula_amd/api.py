@@ -1,0 +1,165 @@
+"""Python mirror of the reference's host-side API for the transport path:
+``octree<true>`` (Geometry), ``gpu_material_manager`` (Materials) and
+``gpu_driver`` (Driver), all calling libnbl_hip.so through its C ABI.
+
+Method names and argument meaning follow the reference
+(geometry/octree.h:42-97, core/gpu_material_manager.h:37-74,
+drivers/gpu/gpu_driver.h:58-182); errors are raised as NblError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import call, ptr
+from . import synth
+
+
+def as_vec3(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    assert a.ndim == 2 and a.shape[1] == 3
+    return a
+
+
+class Geometry:
+    """octree<true>: built on the host with the reference's rules, resident on the device."""
+
+    def __init__(self, triangles: np.ndarray):
+        self.triangles = np.ascontiguousarray(triangles, dtype=L.TRIANGLE_DTYPE)
+        h = C.c_void_p()
+        call("nbl_geometry_create", ptr(self.triangles), len(self.triangles), C.byref(h))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            L.lib.nbl_geometry_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def info(self) -> dict:
+        i = L.GeometryInfo()
+        call("nbl_geometry_get_info", self.h, C.byref(i))
+        return {f: (np.array([getattr(i, f).x, getattr(i, f).y, getattr(i, f).z], np.float32)
+                    if f.startswith("aabb") else getattr(i, f)) for f, _ in L.GeometryInfo._fields_}
+
+    def export_linearized(self):
+        inf = self.info()
+        d = np.zeros(inf["linearized_size"], np.int32)
+        o = np.zeros(inf["n_triangles_unique"], np.uint32)
+        call("nbl_geometry_export_linearized", self.h, ptr(d), ptr(o))
+        return d, o
+
+    def in_domain(self, pos):
+        pos = as_vec3(pos)
+        out = np.zeros(len(pos), np.uint8)
+        call("nbl_geometry_in_domain_batch", self.h, ptr(pos), len(pos), ptr(out))
+        return out
+
+    def propagate(self, pos, dir, dist, ignore_tri=None, ignore_mat=None):
+        pos, dir = as_vec3(pos), as_vec3(dir)
+        n = len(pos)
+        dist = np.ascontiguousarray(dist, np.float32)
+        it = np.full(n, L.NO_TRIANGLE, np.uint32) if ignore_tri is None else np.ascontiguousarray(ignore_tri, np.uint32)
+        im = np.full(n, L.MAT_VACUUM, np.int32) if ignore_mat is None else np.ascontiguousarray(ignore_mat, np.int32)
+        od = np.zeros(n, np.float32); ot = np.zeros(n, np.uint32)
+        call("nbl_geometry_propagate_batch", self.h, ptr(pos), ptr(dir), ptr(dist), ptr(it), ptr(im), n, ptr(od), ptr(ot))
+        return od, ot
+
+
+def octree_linearize(triangles):
+    """linearized_octree(octree_root(triangles)) on the host (no device needed)."""
+    t = np.ascontiguousarray(triangles, dtype=L.TRIANGLE_DTYPE)
+    size = C.c_uint64(); nu = C.c_uint32()
+    call("nbl_octree_linearize", ptr(t), len(t), None, C.byref(size), None, C.byref(nu), None, None)
+    d = np.zeros(size.value, np.int32); o = np.zeros(nu.value, np.uint32)
+    c = np.zeros(3, np.float32); h = np.zeros(3, np.float32)
+    call("nbl_octree_linearize", ptr(t), len(t), ptr(d), C.byref(size), ptr(o), C.byref(nu), ptr(c), ptr(h))
+    return d, o, c, h
+
+
+class PreparedMaterial:
+    """A material's tables after the reference's create() post-processing
+    (nbl_kieft_prepare), held in numpy arrays; .desc points at them."""
+
+    def __init__(self, spec: synth.MaterialSpec):
+        raw = synth.raw_material(spec)
+        self.spec = spec
+        nk, np_ = synth.N_K, synth.N_P
+        self.el_imfp = np.zeros(nk, np.float32)
+        self.el_icdf = np.zeros((nk, np_), np.float32)
+        self.in_imfp = np.zeros(nk, np.float32)
+        self.in_w0 = np.zeros((nk, np_), np.float32)
+        self.ion = np.zeros((nk, np_), np.float32)
+        self.raw_struct = raw.raw_struct(L.KieftRaw)
+        self._raw = raw
+        self.desc = L.MaterialDesc()
+        call("nbl_kieft_prepare", C.byref(self.raw_struct), ptr(self.el_imfp), ptr(self.el_icdf),
+             ptr(self.in_imfp), ptr(self.in_w0), ptr(self.ion), C.byref(self.desc))
+
+
+_PREP: dict = {}
+
+
+def prepared(spec: synth.MaterialSpec) -> PreparedMaterial:
+    if spec.name not in _PREP or _PREP[spec.name].spec != spec:
+        _PREP[spec.name] = PreparedMaterial(spec)
+    return _PREP[spec.name]
+
+
+class Materials:
+    """gpu_material_manager: the listed materials uploaded to the device."""
+
+    def __init__(self, specs):
+        self.prepared = [prepared(s) for s in specs]
+        self.descs = (L.MaterialDesc * len(self.prepared))(*[p.desc for p in self.prepared])
+        h = C.c_void_p()
+        call("nbl_materials_create", self.descs, len(self.prepared), C.byref(h))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            L.lib.nbl_materials_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def max_energy(self) -> float:
+        f = C.c_float()
+        call("nbl_materials_get_max_energy", self.h, C.byref(f))
+        return f.value
+
+    def table1d(self, material, table, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros(len(x), np.float32)
+        call("nbl_table1d_get_batch", self.h, material, table, ptr(x), len(x), ptr(out))
+        return out
+
+    def table2d(self, material, table, x, y, rounddown=False):
+        x = np.ascontiguousarray(x, np.float32); y = np.ascontiguousarray(y, np.float32)
+        out = np.zeros(len(x), np.float32)
+        call("nbl_table2d_get_batch", self.h, material, table, int(rounddown), ptr(x), ptr(y), len(x), ptr(out))
+        return out
+
+    def physics(self, event, particles, material, uniforms, last_tri=None, geometry=None):
+        """One event per particle with injected uniforms; returns a dict of outputs."""
+        n = len(particles)
+        io = np.ascontiguousarray(particles, L.PARTICLE_DTYPE).copy()
+        mat = np.ascontiguousarray(material, np.int32)
+        u = np.ascontiguousarray(uniforms, np.float32).reshape(n, L.NUM_UNIFORMS)
+        lt = None if last_tri is None else np.ascontiguousarray(last_tri, np.uint32)
+        om = np.zeros(n, np.int32); st = np.zeros(n, np.uint8)
+        sec = np.zeros(n, L.PARTICLE_DTYPE); has = np.zeros(n, np.uint8)
+        sc = np.zeros(n, np.float32); ty = np.zeros(n, np.uint8)
+        call("nbl_physics_batch", self.h, geometry.h if geometry is not None else None, event, ptr(mat), ptr(lt),
+             ptr(u), n, ptr(io), ptr(om), ptr(st), ptr(sec), ptr(has), ptr(sc), ptr(ty))
+        return dict(particles=io, material=om, status=st, secondary=sec, has_secondary=has, scalar=sc, type=ty)
+
+
+def device_info():
+    name = C.create_string_buffer(128)
+    cu = C.c_int()
+    call("nbl_device_info", name, 128, C.byref(cu))
+    return name.value.decode(), cu.value

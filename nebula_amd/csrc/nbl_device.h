@@ -1,0 +1,448 @@
+// nbl_device.h -- device-side data layout and inline functions of the
+// MI355X electron-transport path: octree traversal, Moller-Trumbore tests,
+// table interpolation, XORWOW streams and the Kieft-Bosch / boundary physics.
+//
+// Numeric contract (shared with every kernel of the library): IEEE float32,
+// correctly rounded division and square root, no fused multiply-add (the
+// library is built with -ffp-contract=off), and the operation order of the
+// reference, so that traversal and table interpolation are bit-identical to
+// the reference's CPU code.  Transcendentals (log, exp, expm1, sin, cos,
+// atan2) come from the ROCm device library and may differ from the host libm
+// by an ulp or two.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nbl {
+
+#define NBL_HD __host__ __device__ __forceinline__
+
+constexpr float EPSILON = 1.1920928955078125e-06f;   // 10*FLT_EPSILON, config/data_types.h:94
+constexpr float PI_F = 3.1415926535897932f;            // config/config.h:32
+constexpr uint32_t NO_TRI = 0xFFFFFFFFu;
+constexpr uint32_t MAX_CELLS = 1u << 20;               // cell-walk bound (never reached by a terminating walk)
+
+// Special materials, core/cpu_material_manager.h:82-91
+constexpr int MAT_NOP = -128, MAT_TERMINATOR = -127, MAT_DETECTOR = -126,
+              MAT_DETECTOR_LT50 = -125, MAT_DETECTOR_GE50 = -124, MAT_VACUUM = -123,
+              MAT_MIRROR = -122;
+
+struct f3 { float x, y, z; };
+NBL_HD f3 operator+(f3 a, f3 b) { return { a.x + b.x, a.y + b.y, a.z + b.z }; }
+NBL_HD f3 operator-(f3 a, f3 b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
+NBL_HD f3 operator*(f3 a, float b) { return { a.x * b, a.y * b, a.z * b }; }
+NBL_HD f3 operator*(float a, f3 b) { return { a * b.x, a * b.y, a * b.z }; }
+NBL_HD f3 cross(f3 a, f3 b) { return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x }; }
+NBL_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// normalise / normalised in their CPU form, vector_math.h:128-158: a * (1 / |a|)
+NBL_HD f3 normalised(f3 a) { const float r = 1 / sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); return a * r; }
+
+// ---------------------------------------------------------------------------
+// Geometry layout in HBM
+//   nodes : 8 x uint64 child words per internal octree node (one 64-B line);
+//           node 0 is the root.  A child word is either an internal node
+//           index (bit 63 clear) or a leaf (bit 63 set) with the leaf's first
+//           record in bits [0,32) and its record count in bits [32,63).
+//   recs  : 48-B triangle records, stored once per leaf reference, in the
+//           leaf's triangle order, so a leaf is one contiguous run.
+// ---------------------------------------------------------------------------
+constexpr uint64_t LEAF_BIT = 1ull << 63;
+
+struct __attribute__((aligned(16))) tri_rec {
+    float r0x, r0y, r0z, e1x;
+    float e1y, e1z, e2x, e2y;
+    float e2z; uint32_t mats;  // int16 material_in | int16 material_out << 16
+    uint32_t id;               // index of the triangle in the create() input
+    uint32_t pad;
+};
+static_assert(sizeof(tri_rec) == 48, "tri_rec is 48 bytes");
+
+struct dgeom {
+    const uint64_t* nodes;
+    const tri_rec* recs;
+    float cx, cy, cz;          // root centre
+    float hx, hy, hz;          // root half size
+    float max_extent;          // 2*|halfsize|, octree.inl:300
+};
+
+struct prop_counters { uint32_t node_steps, tri_tests; };
+
+NBL_HD bool in_domain(const dgeom& g, f3 p)   // octree.inl:24-30
+{
+    return (p.x > g.cx - g.hx) && (p.x < g.cx + g.hx) && (p.y > g.cy - g.hy) &&
+           (p.y < g.cy + g.hy) && (p.z > g.cz - g.hz) && (p.z < g.cz + g.hz);
+}
+
+// triangle::intersect_ray, core/triangle.inl:7-28
+__device__ __forceinline__ float intersect_ray(f3 r0, f3 e1, f3 e2, f3 start, f3 dir)
+{
+    const f3 pvec = cross(dir, e2);
+    const float det = dot(e1, pvec);
+    if (fabsf(det) < EPSILON) return -1;
+    const f3 tvec = start - r0;
+    const float u = dot(tvec, pvec) / det;
+    if ((u < -EPSILON) || (u > 1 + EPSILON)) return -1;
+    const f3 qvec = cross(tvec, e1);
+    const float v = dot(dir, qvec) / det;
+    if ((v < -EPSILON) || (u + v > 1 + EPSILON)) return -1;
+    const float t = dot(e2, qvec) / det;
+    if (t < 0) return -1;
+    return t;
+}
+
+// octree::propagate, geometry/octree.inl:32-192.  Same cell walk (location
+// codes, re-descent from the root, neighbour by bit flip), same AABB and
+// intersection arithmetic and the same "t <= best + EPSILON" replacement
+// rule over each leaf's triangles in leaf order.  Returns the travelled
+// distance; *hit gets the triangle's input index or NO_TRI.
+template<bool COUNT>
+__device__ __forceinline__ float propagate(const dgeom& g, f3 start, f3 dir, float distance,
+    uint32_t ignore_tri, int ignore_mat, uint32_t* hit, prop_counters* cnt)
+{
+    *hit = NO_TRI;
+    f3 pos = start;
+    float to_go = distance;
+    uint64_t location = 1;
+    uint32_t cells = 0;
+    do {
+        float ax = g.cx, ay = g.cy, az = g.cz, aw = 1;
+        uint32_t node = 0;
+        uint64_t child = 0;
+        bool leaf = false;
+        // descend to the node named by the location code
+        for (int i = 60 - __clzll((long long)location); i >= 0; i -= 3) {
+            const unsigned oct = (unsigned)(location >> i) & 7u;
+            child = g.nodes[(size_t)node * 8 + oct];
+            ax += aw * g.hx * ((float)(oct & 1u) - 0.5f);
+            ay += aw * g.hy * ((float)((oct & 2u) >> 1) - 0.5f);
+            az += aw * g.hz * ((float)((oct & 4u) >> 2) - 0.5f);
+            aw *= 0.5f;
+            if (COUNT) ++cnt->node_steps;
+            leaf = (child & LEAF_BIT) != 0;
+            node = (uint32_t)child;
+            if (leaf) break;
+        }
+        // descend by position to the leaf containing the point
+        while (!leaf) {
+            unsigned oct = 0;
+            oct ^= (pos.x > ax) ? 1u : 0u;
+            oct ^= (pos.y > ay) ? 2u : 0u;
+            oct ^= (pos.z > az) ? 4u : 0u;
+            location = (location << 3) | oct;
+            child = g.nodes[(size_t)node * 8 + oct];
+            ax += aw * g.hx * ((float)(oct & 1u) - 0.5f);
+            ay += aw * g.hy * ((float)((oct & 2u) >> 1) - 0.5f);
+            az += aw * g.hz * ((float)((oct & 4u) >> 2) - 0.5f);
+            aw *= 0.5f;
+            if (COUNT) ++cnt->node_steps;
+            leaf = (child & LEAF_BIT) != 0;
+            node = (uint32_t)child;
+        }
+        const uint32_t first = (uint32_t)child;
+        const uint32_t count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
+
+        // exit plane of the leaf cell (AABB_intersect, octree.inl:211-220)
+        float best;
+        int axis;          // -1, -2, -4 for x, y, z; 0 once a triangle is kept
+        uint32_t best_id = NO_TRI;
+        {
+            const float sx = aw * g.hx, sy = aw * g.hy, sz = aw * g.hz;
+            const float tx = (ax + copysignf(sx + EPSILON, dir.x) - pos.x) / dir.x;
+            const float ty = (ay + copysignf(sy + EPSILON, dir.y) - pos.y) / dir.y;
+            const float tz = (az + copysignf(sz + EPSILON, dir.z) - pos.z) / dir.z;
+            if ((tx < ty) && (tx < tz)) { best = tx; axis = -1; }
+            else if ((ty < tx) && (ty < tz)) { best = ty; axis = -2; }
+            else { best = tz; axis = -4; }
+        }
+        if (COUNT) cnt->node_steps += count + 1;
+        for (uint32_t k = first; k < first + count; ++k) {
+            const float4* rp = reinterpret_cast<const float4*>(g.recs + k);
+            const float4 a = rp[0], b = rp[1], c = rp[2];
+            const uint32_t id = __float_as_uint(c.z);
+            if (id == ignore_tri) continue;
+            const f3 r0{ a.x, a.y, a.z }, e1{ a.w, b.x, b.y }, e2{ b.z, b.w, c.x };
+            const uint32_t mats = __float_as_uint(c.y);
+            const int mat_in = (int)(int16_t)(mats & 0xFFFFu);
+            const int mat_out = (int)(int16_t)(mats >> 16);
+            const int mo = dot(cross(e1, e2), dir) < 0 ? mat_in : mat_out;
+            if (mo == ignore_mat) continue;
+            if (COUNT) ++cnt->tri_tests;
+            const float t = intersect_ray(r0, e1, e2, pos, dir);
+            if ((t > 0) && (t <= best + EPSILON)) { best = t; axis = 0; best_id = id; }
+        }
+        if (best >= to_go) return distance;
+        if (axis == 0) { *hit = best_id; return distance - to_go + best; }
+        to_go -= best;
+        pos = pos + dir * best;
+        const unsigned mask = (unsigned)(-axis);
+        unsigned value;
+        if (mask == 1) value = (dir.x >= 0) ? 0u : 1u;
+        else if (mask == 2) value = (dir.y >= 0) ? 0u : 2u;
+        else value = (dir.z >= 0) ? 0u : 4u;
+        while (location > 1) {
+            if ((location & mask) == value) { location ^= mask; break; }
+            location >>= 3;
+        }
+        if (++cells >= MAX_CELLS) return distance;
+    } while (location > 1);
+    return distance;
+}
+
+// ---------------------------------------------------------------------------
+// Tables: table_1D::get, table_2D::get / get_rounddown (table_1D.inl:65-81,
+// table_2D.inl:73-104).  2D storage (i, j) -> d[i*h + j], x = energy axis.
+// ---------------------------------------------------------------------------
+struct dtab1 { const float* d; int n; float x_min, x_step; };
+struct dtab2 { const float* d; int w, h; float x_min, x_step, y_min, y_step; };
+
+NBL_HD int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+NBL_HD float clampr(float i, float lo, float hi) { return i < lo ? lo : (i > hi ? hi : i); }
+// std::min / std::max argument conventions of the reference's CPU build
+NBL_HD float minr(float a, float b) { return (b < a) ? b : a; }
+NBL_HD float maxr(float a, float b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ float tab1_get(const dtab1& t, float x)
+{
+    const float xi = (x - t.x_min) * t.x_step;
+    const int lo = clampi((int)floorf(xi), 0, t.n - 2);
+    const float v0 = t.d[lo], v1 = t.d[lo + 1];
+    const float f = xi - (float)lo;
+    return (1 - f) * v0 + f * v1;
+}
+__device__ __forceinline__ float tab2_get(const dtab2& t, float x, float y)
+{
+    const float xi = (x - t.x_min) * t.x_step;
+    const float yi = (y - t.y_min) * t.y_step;
+    const int lx = clampi((int)floorf(xi), 0, t.w - 2);
+    const int ly = clampi((int)floorf(yi), 0, t.h - 2);
+    const float* r0 = t.d + (size_t)lx * t.h + ly;
+    const float* r1 = r0 + t.h;
+    const float v00 = r0[0], v01 = r0[1], v10 = r1[0], v11 = r1[1];
+    const float fx = xi - (float)lx, fy = yi - (float)ly;
+    return (1 - fx) * (1 - fy) * v00 + fx * (1 - fy) * v10 + (1 - fx) * fy * v01 + fx * fy * v11;
+}
+__device__ __forceinline__ float tab2_rounddown(const dtab2& t, float x, float y)
+{
+    const float xi = (x - t.x_min) * t.x_step;
+    const float yi = (y - t.y_min) * t.y_step;
+    const int lx = clampi((int)floorf(xi), 0, t.w - 1);
+    const int ly = clampi((int)floorf(yi), 0, t.h - 1);
+    return t.d[(size_t)lx * t.h + ly];
+}
+
+// Material as held on the device (material<scatter_list<kieft_inelastic, kieft_elastic>>).
+struct dmat {
+    float barrier, fermi, band_gap, phonon_loss, recoil_const;
+    dtab1 el_imfp, in_imfp;
+    dtab2 el_icdf, in_icdf, ion;
+};
+
+// ---------------------------------------------------------------------------
+// XORWOW (Marsaglia, J. Stat. Softw. 8(14), 2003, sec. 3.1) with the
+// project's seeding: splitmix64 (Steele, Lea & Flood 2014) of the seed and
+// the sequence number fills the five xorshift words and the Weyl counter.
+// Uniforms: ((r >> 8) + 1) * 2^-24, in (0, 1].
+// ---------------------------------------------------------------------------
+struct rng_state { uint32_t x, y, z, w, v, d; };
+
+NBL_HD uint64_t splitmix64_next(uint64_t& s)
+{
+    uint64_t r = (s += 0x9E3779B97F4A7C15ull);
+    r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+    r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+    return r ^ (r >> 31);
+}
+NBL_HD rng_state rng_seed(uint64_t seed, uint64_t sequence)
+{
+    uint64_t s = seed ^ (sequence * 0xD1B54A32D192ED03ull);
+    const uint64_t a = splitmix64_next(s), b = splitmix64_next(s), c = splitmix64_next(s);
+    rng_state r{ (uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32), (uint32_t)c, (uint32_t)(c >> 32) };
+    if ((r.x | r.y | r.z | r.w | r.v) == 0) r.x = 0x6C078965u;
+    return r;
+}
+NBL_HD float rng_unit(rng_state& s)
+{
+    const uint32_t t = s.x ^ (s.x >> 2);
+    s.x = s.y; s.y = s.z; s.z = s.w; s.w = s.v;
+    s.v = (s.v ^ (s.v << 4)) ^ (t ^ (t << 1));
+    s.d += 362437u;
+    const uint32_t r = s.d + s.v;
+    return (float)((r >> 8) + 1u) * 5.9604644775390625e-8f;
+}
+
+// A draw source: XORWOW stream, or injected uniforms for unit-parity runs.
+struct rng_xorwow {
+    rng_state s;
+    __device__ __forceinline__ float unit() { return rng_unit(s); }
+};
+struct rng_inject {
+    const float* u; int k;
+    __device__ __forceinline__ float unit() { return k < 8 ? u[k++] : 0.5f; }
+};
+
+// random_generator draws, common/util/random.h:64-83.  uniform_vector takes
+// cos(theta) first, then phi (the reference leaves the order to the compiler).
+template<typename R> __device__ __forceinline__ float rng_phi(R& r) { return 2 * PI_F * r.unit(); }
+template<typename R> __device__ __forceinline__ float rng_exponential(R& r, float tau) { return -tau * logf(r.unit()); }
+
+// make_normal_vec, vector_math.h:171-186
+__device__ __forceinline__ f3 make_normal_vec(f3 dir, float phi)
+{
+    const float az = atan2f(dir.y, dir.x);
+    const float sin_az = sinf(az), cos_az = cosf(az);
+    const f3 unit_v{ dir.z * cos_az, dir.z * sin_az, -sqrtf(dir.x * dir.x + dir.y * dir.y) };
+    const f3 unit_u = cross(unit_v, dir);
+    const float sin_phi = sinf(phi), cos_phi = cosf(phi);
+    return unit_u * cos_phi + unit_v * sin_phi;
+}
+// make_unit_vec, vector_math.h:194-206
+__device__ __forceinline__ f3 make_unit_vec(float cos_theta, float phi)
+{
+    const float sin_phi = sinf(phi), cos_phi = cosf(phi);
+    cos_theta = clampr(cos_theta, -1, 1);
+    const float sin_theta = sqrtf(1 - cos_theta * cos_theta);
+    return { sin_theta * cos_phi, sin_theta * sin_phi, cos_theta };
+}
+template<typename R> __device__ __forceinline__ f3 rng_uniform_vector(R& r)
+{
+    const float c = 2 * r.unit() - 1;
+    const float p = rng_phi(r);
+    return make_unit_vec(c, p);
+}
+
+struct dparticle { f3 pos, dir; float e; };
+
+// scatter_list::sample_path (scatter_list.h:106-112, 148-162): the elastic
+// mechanism (type 2) is sampled first and wins ties; inelastic is type 1.
+template<typename R>
+__device__ __forceinline__ float sample_path(const dmat& m, float kin_energy, R& rng, uint8_t* type)
+{
+    const float le = logf(kin_energy);
+    float best = __builtin_inff();
+    uint8_t ty = 0;
+    {
+        const float imfp = expf(tab1_get(m.el_imfp, le));
+        const float path = rng_exponential(rng, 1 / imfp);
+        if (path < best) { best = path; ty = 2; }
+    }
+    {
+        const float imfp = expf(tab1_get(m.in_imfp, le));
+        const float path = rng_exponential(rng, 1 / imfp);
+        if (path < best) { best = path; ty = 1; }
+    }
+    *type = ty;
+    return best;
+}
+
+// kieft_elastic::execute, physics/kieft/elastic.h:67-115 (phonon + recoil loss on)
+template<typename R>
+__device__ __forceinline__ void elastic_execute(const dmat& m, dparticle& p, R& rng)
+{
+    const float x = logf(p.e);
+    const float y = rng.unit();
+    const float cos_theta = clampr(tab2_get(m.el_icdf, x, y), -1, 1);
+    const float sin_theta = sqrtf(1 - cos_theta * cos_theta);
+    p.dir = normalised(p.dir);
+    const f3 nd = normalised(make_normal_vec(p.dir, rng_phi(rng)));
+    p.dir = p.dir * cos_theta + nd * sin_theta;
+    if (p.e < 200) p.e -= minr(50e-3f, m.phonon_loss);
+    else p.e -= m.recoil_const * (1 - cos_theta) * p.e;
+}
+
+// kieft_inelastic::execute, physics/kieft/inelastic.h:68-206 (all options on).
+// Returns true and fills *sec when a secondary electron is created.
+template<typename R>
+__device__ __forceinline__ bool inelastic_execute(const dmat& m, dparticle& p, R& rng, dparticle* sec)
+{
+    const float omega0 = expf(tab2_get(m.in_icdf, logf(p.e), rng.unit()));
+    float binding = tab2_rounddown(m.ion, logf(omega0), rng.unit());
+    float omega;
+    {
+        const float omega_max = 0.5f * (p.e + omega0 - m.fermi);
+        float omega_min = omega0;
+        float w0 = minr(omega0 - 1, maxr(0, binding) - m.fermi);
+        if (p.e > 2 * omega0) {
+            omega_min = 0.5f * (p.e + omega0 - sqrtf(p.e * (p.e - 2 * omega0)));
+            w0 = omega0;
+        }
+        const float U = rng.unit();
+        if ((w0 > 0) && (omega_min > w0) && (omega_min < omega_max)) {
+            const float f_min = 1 / w0 * logf((omega_min - w0) / omega_min);
+            const float f_max = 1 / w0 * logf((omega_max - w0) / omega_max);
+            omega = -w0 / expm1f(w0 * (f_min * (1 - U) + f_max * U));
+        } else {
+            omega = omega0 * omega_max / (omega0 * (1 - U) + omega_max * U);
+        }
+    }
+    if (binding < 0) {
+        if (m.band_gap < 0) {
+            // metals: nothing (inelastic.h:132-134)
+        } else if (omega0 > m.band_gap) {
+            binding = m.band_gap;
+        } else {
+            p.e -= omega0;               // optical phonon loss, inelastic.h:140-149
+            return false;
+        }
+    }
+    binding = maxr(0, binding);
+    p.dir = normalised(p.dir);
+    const f3 nd = normalised(make_normal_vec(p.dir, rng_phi(rng)));
+    const float K = p.e - m.fermi + 2 * binding;
+    const float dK = binding + omega;
+    const float cos_theta = sqrtf(dK / K);
+    const float sin_theta = sqrtf(1 - clampr(cos_theta * cos_theta, 0, 1));
+    f3 sdir = p.dir * cos_theta + nd * sin_theta;
+    sdir = normalised(sdir);
+    const float sb = sqrtf(binding / dK);
+    sdir = sdir + sb * rng_uniform_vector(rng);
+    sdir = normalised(sdir);
+    sec->e = m.fermi + omega - binding;
+    sec->pos = p.pos;
+    sec->dir = sdir;
+    p.e -= omega;
+    p.dir = p.dir - cos_theta * sdir;
+    return true;
+}
+
+constexpr int OUT_ALIVE = 0, OUT_DETECTED = 1, OUT_TERMINATED = 2;
+
+// boundary_intersect<true,true,false>::execute, physics/boundary_intersect.h:39-156.
+template<typename R>
+__device__ __forceinline__ int intersect_execute(const dmat* mats, const tri_rec& tri, dparticle& p, int& mat, R& rng)
+{
+    const f3 nd = normalised(p.dir);
+    const f3 e1{ tri.e1x, tri.e1y, tri.e1z }, e2{ tri.e2x, tri.e2y, tri.e2z };
+    const f3 n = normalised(cross(e1, e2));
+    const float cos_theta = dot(n, nd);
+    const int t_in = (int)(int16_t)(tri.mats & 0xFFFFu), t_out = (int)(int16_t)(tri.mats >> 16);
+    int mat_in, mat_out;
+    if (cos_theta > 0) { mat_in = t_in; mat_out = t_out; }
+    else { mat_in = t_out; mat_out = t_in; }
+    switch (mat_out) {
+    case MAT_DETECTOR: return OUT_DETECTED;
+    case MAT_DETECTOR_LT50: return p.e < 50 ? OUT_DETECTED : OUT_ALIVE;
+    case MAT_DETECTOR_GE50: return p.e >= 50 ? OUT_DETECTED : OUT_ALIVE;
+    case MAT_TERMINATOR: return OUT_TERMINATED;
+    case MAT_MIRROR: p.dir = nd - 2 * n * cos_theta; return OUT_ALIVE;
+    case MAT_NOP: return OUT_ALIVE;
+    default: break;
+    }
+    float dU = 0;
+    if (mat_in >= 0) dU -= mats[mat_in].barrier;
+    if (mat_out >= 0) dU += mats[mat_out].barrier;
+    if (p.e * cos_theta * cos_theta + dU > 0) {
+        const float s = sqrtf(1 + dU / (p.e * cos_theta * cos_theta));
+        const float T = 4 * s / ((1 + s) * (1 + s));
+        if (rng.unit() < T) {
+            p.dir = (nd - n * cos_theta) + s * n * cos_theta;
+            p.e += dU;
+            mat = mat_out;
+            return OUT_ALIVE;
+        }
+    }
+    p.dir = nd - 2 * n * cos_theta;
+    return OUT_ALIVE;
+}
+
+}  // namespace nbl

@@ -1,0 +1,199 @@
+"""Synthetic cross-section tables in the raw form a cross-section tool writes.
+
+There is no network and no cstool output here, so every workload uses
+materials generated from closed-form models, fixed once in ``MANIFEST``:
+
+* elastic: screened Rutherford,
+  dsigma/dOmega = (1.44 eV nm * Z / (2E))^2 / (1 - cos(theta) + 2 eta)^2,
+  eta = 3.4 Z^(2/3) / E[eV]; total sigma = pi (1.44 Z / 2E)^2 / (eta (1 + eta));
+  the ICDF has the closed form cos(theta) = 1 - 2 eta u / (1 + eta - u).
+* inelastic: single-pole Drude energy-loss function
+  ELF(w) = wp^2 g w / ((wp^2 - w^2)^2 + (g w)^2), optical approximation
+  dlambda^-1/dw = ELF(w) ln(q+/q-) / (2 pi E a0) with ln(q+/q-) =
+  ln((1 + sqrt(1 - w/E))^2 / (w/E)), w in (w_lo, E - E_F).
+* binding ICDF: inner shells ionised with probabilities rising as
+  f (1 - B/w') above each edge B; the remainder is "no inner shell" (-1).
+
+Grids: 1024 log-spaced kinetic energies from 1 eV to 10 keV and 1024
+probabilities from 0 to 1, float32.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HARTREE_EV = 27.211386245988
+BOHR_NM = 0.0529177210903
+AVOGADRO = 6.02214076e23
+
+N_K = 1024
+N_P = 1024
+K_MIN, K_MAX = 1.0, 1.0e4
+P_MIN, P_MAX = 0.0, 1.0
+
+
+@dataclass
+class MaterialSpec:
+    name: str
+    z: float                  # screening charge of the elastic model
+    density: float            # atoms per nm^3
+    plasmon: float            # Drude pole, eV
+    damping: float            # Drude width, eV
+    fermi: float              # eV
+    work_function: float      # eV; barrier = fermi + work_function
+    band_gap: float           # eV; < 0 for a metal
+    phonon_loss: float        # eV
+    atomic_mass: float        # g/mol of the effective atom
+    inner_shells: list = field(default_factory=list)   # [(binding eV, weight)]
+    outer_shells: list = field(default_factory=list)   # eV
+
+    @property
+    def barrier(self) -> float:
+        return self.fermi + self.work_function
+
+
+SI = MaterialSpec("Si", z=14, density=49.94, plasmon=16.7, damping=4.0, fermi=7.8,
+                  work_function=4.8, band_gap=1.12, phonon_loss=0.063, atomic_mass=28.0855,
+                  inner_shells=[(99.2, 0.6), (149.7, 0.2), (1839.0, 0.5)], outer_shells=[7.0])
+PMMA = MaterialSpec("PMMA", z=3.6, density=107.4, plasmon=19.0, damping=6.0, fermi=0.0,
+                    work_function=4.0, band_gap=5.6, phonon_loss=0.1, atomic_mass=6.67,
+                    inner_shells=[(284.2, 0.5), (543.1, 0.3)], outer_shells=[8.0])
+# third material of the batched scan: a metal-like layer
+W_LIKE = MaterialSpec("W-like", z=74, density=63.2, plasmon=22.0, damping=8.0, fermi=9.0,
+                      work_function=4.5, band_gap=-1.0, phonon_loss=0.03, atomic_mass=183.84,
+                      inner_shells=[(245.0, 0.3), (1809.0, 0.3)], outer_shells=[3.0, 8.0])
+
+
+def stripe_materials(n=8):
+    """n materials with Drude pole 10..25 eV, Fermi 0..10 eV, Z 6..79."""
+    out = []
+    for k in range(n):
+        f = k / max(n - 1, 1)
+        fermi = 10.0 * f
+        out.append(MaterialSpec(
+            f"stripe{k}", z=round(6 + 73 * f), density=50.0, plasmon=10.0 + 15.0 * f,
+            damping=3.0 + 3.0 * f, fermi=fermi, work_function=4.5,
+            band_gap=-1.0 if fermi > 5 else 2.0, phonon_loss=0.05,
+            atomic_mass=2.0 * round(6 + 73 * f),
+            inner_shells=[(120.0 + 200.0 * f, 0.4)], outer_shells=[6.0]))
+    return out
+
+
+MANIFEST = {
+    "grid": {"n_k": N_K, "k_min_eV": K_MIN, "k_max_eV": K_MAX, "n_p": N_P, "p": [P_MIN, P_MAX]},
+    "elastic": "screened Rutherford, eta = 3.4 Z^(2/3)/E[eV]",
+    "inelastic": "single-pole Drude ELF, optical approximation, 1/(2 pi E a0)",
+    "materials": {m.name: m.__dict__ for m in (SI, PMMA, W_LIKE)},
+}
+
+
+def energy_grid():
+    ks = np.log(K_MAX / K_MIN) / (N_K - 1)
+    return K_MIN * np.exp(np.arange(N_K) * ks)
+
+
+def prob_grid():
+    return P_MIN + np.arange(N_P) * ((P_MAX - P_MIN) / (N_P - 1))
+
+
+def screening(spec: MaterialSpec, e):
+    return 3.4 * spec.z ** (2.0 / 3.0) / e
+
+
+def elastic_tables(spec: MaterialSpec):
+    k = energy_grid()
+    p = prob_grid()
+    eta = screening(spec, k)
+    sigma = np.pi * (1.44 * spec.z / (2.0 * k)) ** 2 / (eta * (1.0 + eta))   # nm^2
+    imfp = (spec.density * sigma).astype(np.float32)                         # nm^-1
+    e = eta[:, None]
+    u = p[None, :]
+    cos_t = 1.0 - 2.0 * e * u / (1.0 + e - u)
+    return imfp, cos_t.astype(np.float32)
+
+
+def drude_elf(spec: MaterialSpec, w):
+    wp, g = spec.plasmon, spec.damping
+    return wp * wp * g * w / ((wp * wp - w * w) ** 2 + (g * w) ** 2)
+
+
+def inelastic_tables(spec: MaterialSpec, n_w=2048, w_lo=0.05):
+    k = energy_grid()
+    p = prob_grid()
+    imfp = np.full(N_K, 1e-10, dtype=np.float64)
+    icdf = np.empty((N_K, N_P), dtype=np.float64)
+    for i, e in enumerate(k):
+        w_hi = e - spec.fermi
+        if w_hi <= w_lo * 1.01:
+            icdf[i, :] = w_lo
+            continue
+        w = np.geomspace(w_lo, w_hi, n_w)
+        x = np.minimum(w / e, 1.0)
+        lq = np.log((1.0 + np.sqrt(1.0 - x)) ** 2 / x)
+        dens = drude_elf(spec, w) * lq / (2.0 * np.pi * (e / HARTREE_EV) * BOHR_NM) / HARTREE_EV  # nm^-1 eV^-1
+        seg = 0.5 * (dens[1:] + dens[:-1]) * np.diff(w)
+        cdf = np.concatenate([[0.0], np.cumsum(seg)])
+        total = cdf[-1]
+        if not total > 0:
+            icdf[i, :] = w_lo
+            continue
+        imfp[i] = max(total, 1e-10)
+        icdf[i, :] = np.interp(p * total, cdf, w)
+    return imfp.astype(np.float32), icdf.astype(np.float32)
+
+
+def binding_table(spec: MaterialSpec):
+    """Raw binding ICDF over (omega', P): -1 = no inner shell."""
+    k = energy_grid()
+    p = prob_grid()
+    out = np.full((N_K, N_P), -1.0, dtype=np.float64)
+    shells = sorted(spec.inner_shells)
+    for i, w in enumerate(k):
+        open_ = [(b, f * (1.0 - b / w)) for b, f in shells if w > b]
+        if not open_:
+            continue
+        q = np.array([x for _, x in open_])
+        total = q.sum()
+        if total > 1.0:
+            q, total = q / total, 1.0
+        # P in [0, 1-total) -> no inner shell; then one interval per open shell
+        bounds = np.concatenate([[1.0 - total], 1.0 - total + np.cumsum(q)])
+        labels = np.array([-1.0] + [b for b, _ in open_])
+        idx = np.minimum(np.searchsorted(bounds, p, side="right"), len(open_))
+        out[i] = labels[idx]
+    return out.astype(np.float32)
+
+
+class RawMaterial:
+    """Raw tables of one material, kept alive for the C struct that points at them."""
+
+    def __init__(self, spec: MaterialSpec):
+        self.spec = spec
+        self.el_imfp, self.el_cos = elastic_tables(spec)
+        self.in_imfp, self.in_w0 = inelastic_tables(spec)
+        self.binding = binding_table(spec)
+        self.outer = np.ascontiguousarray(spec.outer_shells, dtype=np.float64)
+
+    def raw_struct(self, raw_cls):
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        s = self.spec
+        return raw_cls(
+            barrier=s.barrier, fermi=s.fermi, band_gap=s.band_gap, phonon_loss=s.phonon_loss,
+            effective_a_grams=s.atomic_mass / AVOGADRO, n_k=N_K, k_min=K_MIN, k_max=K_MAX,
+            n_p=N_P, p_min=P_MIN, p_max=P_MAX, elastic_imfp=fp(self.el_imfp),
+            elastic_costheta=fp(self.el_cos), inelastic_imfp=fp(self.in_imfp),
+            inelastic_w0=fp(self.in_w0), binding_icdf=fp(self.binding),
+            n_outer_shells=len(self.outer),
+            outer_shells=self.outer.ctypes.data_as(C.POINTER(C.c_double)))
+
+
+_CACHE: dict = {}
+
+
+def raw_material(spec: MaterialSpec) -> RawMaterial:
+    key = tuple(sorted((k, str(v)) for k, v in spec.__dict__.items()))
+    if key not in _CACHE:
+        _CACHE[key] = RawMaterial(spec)
+    return _CACHE[key]
