@@ -1,0 +1,166 @@
+"""Benchmark: primary electrons/s of the GPU transport on one MI355X, beside
+the CPU oracle on the same host.
+
+Workload (rough surface): a 512 x 512 Gaussian height field (sigma 2 nm,
+correlation length 10 nm, seed 1) over 1 um^2 of synthetic Si, triangulated
+to 522,242 surface triangles (526,334 with walls, bottom and detector); 1e7
+primaries of 300 eV uniform over the 1 um^2 area.  One step = transport of
+all 1e7 primaries to the end of their cascades in one persistent launch.
+
+Output: one JSON line (metric, value, roofline, cpu_baseline, counters).
+`--gpus N` with N > 1 exits: this build measures one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E, AMD public spec
+# algorithmic bytes (one event = one init + event step of one electron)
+BYTES = dict(event=122, lookup_1d=8, lookup_2d=16, rounddown=4, node_step=4, tri_test=44,
+             boundary=52, secondary=65)
+
+
+def sample_top(tris):
+    """Highest vertex of the sample (the detector plane excluded)."""
+    s = tris[tris["material_in"] != -126]
+    z = np.concatenate([s["r0"][:, 2], s["r0"][:, 2] + s["e1"][:, 2], s["r0"][:, 2] + s["e2"][:, 2]])
+    return float(z.max())
+
+
+def algorithmic_bytes(t):
+    return (BYTES["event"] * t["events"] + BYTES["lookup_1d"] * t["lookups_1d"] +
+            BYTES["lookup_2d"] * t["lookups_2d"] + BYTES["rounddown"] * t["rounddowns"] +
+            BYTES["node_step"] * t["node_steps"] + BYTES["tri_test"] * t["tri_tests"] +
+            BYTES["boundary"] * t["boundary_events"] + BYTES["secondary"] * t["secondaries"])
+
+
+def cpu_baseline(tris, spec, energy, n_sample, time_limit, threads):
+    """The oracle's cpu_mt restatement on a bounded prefix of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    import runs
+    import nebula_amd as na
+    og = ol.OrcGeometry(tris)
+    om = ol.OrcMaterials([na.prepared(spec).desc])
+    rng = np.random.default_rng(12345)
+    prim = runs.pencil_primaries(n_sample, energy)
+    prim["pos"][:, 0] = rng.uniform(-500, 500, n_sample)
+    prim["pos"][:, 1] = rng.uniform(-500, 500, n_sample)
+    prim["pos"][:, 2] = sample_top(tris) + 5.0
+    st, _ = runs.oracle_run(og, om, prim, threads=threads, time_limit=time_limit)
+    return st
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--primaries", type=int, default=10_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=600_000)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--stack-depth", type=int, default=0)
+    ap.add_argument("--refill-threshold", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the transport shards by primaries with no "
+              "cross-GPU traffic, and multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import nebula_amd as na
+    from nebula_amd import scenes, synth
+    from nebula_amd.driver import Driver, uniform_beam
+
+    t0 = time.time()
+    tris = scenes.rough_surface()
+    z0 = sample_top(tris) + 5.0      # above the surface, below the detector (top + 15 nm)
+    g = na.Geometry(tris)
+    m = na.Materials([synth.SI])
+    info = g.info()
+    name, cus = na.device_info()
+    print(f"setup: {len(tris)} triangles, {info['n_nodes']} nodes, {info['n_leaves']} leaves, "
+          f"device {name} ({cus} CUs), {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    opts = dict(block_size=args.block_size, blocks_per_cu=args.blocks_per_cu,
+                stack_depth=args.stack_depth, refill_threshold=args.refill_threshold)
+    drv = Driver(g, m, capacity=1024, seed=2024)
+    beam = lambda s: uniform_beam(300.0, args.primaries, (500.0, 500.0), z0, seed=s)
+    for w in range(args.warmup):
+        drv.run_persistent(beam(1000 + w), **opts)
+        print(f"warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    totals = None
+    kernel_ms = []
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        t = drv.run_persistent(beam(s), **opts)
+        kernel_ms.append(t["kernel_ms"])
+        if totals is None:
+            totals = {k: 0 for k in t}
+        for k, v in t.items():
+            totals[k] += v
+        print(f"step {s + 1}/{args.steps}: {t['kernel_ms']:.1f} ms, SE {t['detected_se'] / t['primaries']:.4f} "
+              f"BSE {t['detected_bse'] / t['primaries']:.4f}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t_start
+    assert totals["primaries"] == args.primaries * args.steps, totals
+    ms_per_step = wall * 1000.0 / args.steps
+    value = totals["primaries"] / wall
+    avg_kernel_ms = float(np.mean(kernel_ms))
+    per_launch = {k: v / args.steps for k, v in totals.items()}
+    ach_gbps = algorithmic_bytes(per_launch) / (avg_kernel_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        st = cpu_baseline(tris, synth.SI, 300.0, args.cpu_sample, args.cpu_seconds, threads)
+        cpu = {"value": st["primaries"] / st["seconds"], "unit": "primaries/s", "cores": int(st["threads"]),
+               "kind": "port",
+               "sample": f"oracle cpu_mt restatement, {st['primaries']} primaries of the same workload "
+                         f"(first {args.cpu_sample} of a uniform 300 eV beam, stopped at {args.cpu_seconds:.0f} s), "
+                         f"{st['seconds']:.1f} s, {st['events'] / st['seconds']:.3g} events/s",
+               "events_per_s": st["events"] / st["seconds"],
+               "se_yield": st["detected_se"] / max(st["primaries"], 1),
+               "bse_yield": st["detected_bse"] / max(st["primaries"], 1)}
+        print(f"cpu baseline: {cpu['value']:.0f} primaries/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "primary electrons/s (1 MI355X) vs host CPU; learned vs table sampler at equal yield",
+        "value": value,
+        "unit": "primaries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "rough surface: 512x512 Gaussian height field (sigma 2 nm, corr 10 nm, seed 1), "
+                               "526334 triangles, synthetic Si, 300 eV uniform over 1 um^2",
+                   "primaries_per_step": args.primaries, "triangles": int(len(tris)),
+                   "sampler": "table", "kernel": "persistent"},
+        "roofline": {"bound": "hbm", "achieved": ach_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": ach_gbps / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "k_persistent", "avg_kernel_ms": avg_kernel_ms,
+                     "algorithmic_bytes_per_launch": algorithmic_bytes(per_launch)},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "events_per_s": totals["events"] / wall,
+        "counters_per_step": per_launch,
+        "yields": {"se": totals["detected_se"] / totals["primaries"], "bse": totals["detected_bse"] / totals["primaries"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

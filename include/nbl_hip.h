@@ -321,6 +321,8 @@ typedef struct {
     uint64_t events;         /* init + event steps (one per active particle step)  */
     uint64_t lookups_1d, lookups_2d, rounddowns;
     uint64_t node_steps, tri_tests, boundary_events, secondaries, pending_retries;
+    uint64_t stopped;        /* electrons dropped: 2^20-step cap reached, or a
+                                secondary with no stack space left (normally 0) */
     double   kernel_ms;      /* device time of the transport launch(es)            */
     uint32_t* pixel_se;      /* optional [n_pixels] out (may be NULL)              */
     uint32_t* pixel_bse;     /* optional [n_pixels] out (may be NULL)              */
@@ -330,9 +332,10 @@ typedef struct {
 typedef struct {
     uint32_t block_size;       /* threads per workgroup (64..1024, multiple of 64); 0 = default */
     uint32_t blocks_per_cu;    /* resident workgroups per CU; 0 = default          */
-    uint32_t stack_depth;      /* per-wavefront secondary stack entries; 0 = default */
+    uint32_t stack_depth;      /* per-wavefront secondary stack entries in LDS; 0 = default (128) */
     uint32_t refill_threshold; /* refill a wavefront when >= this many lanes idle; 0 = default (1) */
-    uint32_t reserved[4];
+    uint32_t overflow_depth;   /* further per-wavefront stack entries in HBM; 0 = default (4096) */
+    uint32_t reserved[3];
 } nbl_persistent_opts;
 
 /* Runs every primary of the batch to the end of its cascade in one

@@ -117,14 +117,14 @@ class Tallies(C.Structure):
                 ("events", C.c_uint64), ("lookups_1d", C.c_uint64), ("lookups_2d", C.c_uint64),
                 ("rounddowns", C.c_uint64), ("node_steps", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("boundary_events", C.c_uint64), ("secondaries", C.c_uint64),
-                ("pending_retries", C.c_uint64), ("kernel_ms", C.c_double),
+                ("pending_retries", C.c_uint64), ("stopped", C.c_uint64), ("kernel_ms", C.c_double),
                 ("pixel_se", C.c_void_p), ("pixel_bse", C.c_void_p), ("hist", C.c_void_p)]
 
 
 class PersistentOpts(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("stack_depth", C.c_uint32), ("refill_threshold", C.c_uint32),
-                ("reserved", C.c_uint32 * 4)]
+                ("overflow_depth", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 DETECT_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_uint32, C.c_void_p)

@@ -11,13 +11,35 @@ struct nbl_geometry_s {
     built_octree host;
     dbuf<uint64_t> nodes;
     dbuf<tri_rec> recs;
+    dbuf<uint32_t> id_rec;
     dgeom view{};
     uint32_t n_in = 0;
+    std::vector<uint8_t> has_rec;    // lazily built: id has a record
 };
 
 namespace nbl {
 const dgeom& geometry_view(nbl_geometry g) { return g->view; }
 const tri_rec* geometry_host_recs(nbl_geometry g) { return g->host.recs.data(); }
+uint32_t geometry_n_input(nbl_geometry g) { return g->n_in; }
+// true if triangle id has a device record (not dropped as a duplicate)
+bool geometry_has_record(nbl_geometry g, uint32_t id)
+{
+    if (g->has_rec.empty()) {
+        g->has_rec.assign(g->n_in, 0);
+        for (const tri_rec& r : g->host.recs) g->has_rec[r.id] = 1;
+    }
+    return id < g->n_in && g->has_rec[id];
+}
+// largest material index used by any kept triangle (-1 if none is physical)
+int geometry_max_material(nbl_geometry g)
+{
+    int mx = -1;
+    for (const tri_rec& r : g->host.recs) {
+        mx = std::max(mx, (int)(int16_t)(r.mats & 0xFFFFu));
+        mx = std::max(mx, (int)(int16_t)(r.mats >> 16));
+    }
+    return mx;
+}
 }
 
 namespace {
@@ -68,9 +90,14 @@ int nbl_geometry_create(const nbl_triangle* tris, uint32_t n, nbl_geometry* out)
         g->nodes.upload(g->host.nodes.data(), g->host.nodes.size());
         g->recs.alloc(std::max<size_t>(g->host.recs.size(), 1));
         g->recs.upload(g->host.recs.data(), g->host.recs.size());
+        // one record per input id (NO_TRI for triangles dropped as duplicates)
+        std::vector<uint32_t> id_rec(n, NO_TRI);
+        for (size_t k = g->host.recs.size(); k-- > 0;) id_rec[g->host.recs[k].id] = (uint32_t)k;
+        g->id_rec.alloc(n);
+        g->id_rec.upload(id_rec.data(), n);
         NBL_HIP(hipDeviceSynchronize());
         dgeom& v = g->view;
-        v.nodes = g->nodes.p; v.recs = g->recs.p;
+        v.nodes = g->nodes.p; v.recs = g->recs.p; v.id_rec = g->id_rec.p;
         v.cx = g->host.center[0]; v.cy = g->host.center[1]; v.cz = g->host.center[2];
         v.hx = g->host.halfsize[0]; v.hy = g->host.halfsize[1]; v.hz = g->host.halfsize[2];
         v.max_extent = g->host.max_extent;

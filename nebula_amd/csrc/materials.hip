@@ -312,7 +312,6 @@ int nbl_physics_batch(nbl_materials m, nbl_geometry g, int32_t event, const int3
         require(n == 0 || (material && uniforms && io), "nbl_physics_batch: NULL array");
         if (n == 0) return;
         const int nm = (int)m->views.size();
-        std::vector<dmat> unused;
         for (uint32_t i = 0; i < n; ++i) {
             if (event == NBL_EVENT_INTERSECT) {
                 require(material[i] < nm, "nbl_physics_batch: material index out of range");
@@ -330,8 +329,12 @@ int nbl_physics_batch(nbl_materials m, nbl_geometry g, int32_t event, const int3
             std::vector<tri_rec> by_id(info.n_triangles_in);
             std::vector<uint8_t> have(info.n_triangles_in, 0);
             for (uint32_t k = 0; k < info.n_leaf_refs; ++k) { by_id[recs[k].id] = recs[k]; have[recs[k].id] = 1; }
-            for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t i = 0; i < n; ++i) {
                 require(last_tri[i] < info.n_triangles_in && have[last_tri[i]], "nbl_physics_batch: bad last_tri");
+                const tri_rec& t = by_id[last_tri[i]];
+                const int a = (int16_t)(t.mats & 0xFFFFu), b = (int16_t)(t.mats >> 16);
+                require(a < nm && b < nm, "nbl_physics_batch: triangle material index beyond the material count");
+            }
             tris.alloc(by_id.size());
             tris.upload(by_id.data(), by_id.size());
             dlast.alloc(n);

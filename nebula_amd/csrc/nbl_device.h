@@ -61,6 +61,7 @@ static_assert(sizeof(tri_rec) == 48, "tri_rec is 48 bytes");
 struct dgeom {
     const uint64_t* nodes;
     const tri_rec* recs;
+    const uint32_t* id_rec;    // input triangle id -> index of one of its records
     float cx, cy, cz;          // root centre
     float hx, hy, hz;          // root half size
     float max_extent;          // 2*|halfsize|, octree.inl:300

@@ -1,0 +1,351 @@
+// persistent.hip -- nbl_driver_run_persistent: every primary of a pixel
+// batch runs to the end of its cascade in ONE launch.
+//
+// Replaces the prescan + frame loop of gpu_main.cu:108-213 (push, sort,
+// do_iteration x frame_size, flush) for throughput.  Design:
+//   * each lane of a resident wavefront holds one electron in registers
+//     (position, direction, energy, material, last triangle, pixel tag);
+//     a lane's XORWOW stream persists across the electrons it handles;
+//   * each wavefront keeps a LIFO stack of secondaries in LDS; a lane whose
+//     electron ended pops a secondary first, otherwise the wavefront takes a
+//     contiguous range of primaries with ONE atomic on a global counter
+//     (compaction: idle lanes are refilled at the top of every step, or when
+//     at least refill_threshold lanes are idle);
+//   * an inelastic event may run only if the stack has room for the
+//     secondary it may create; otherwise the electron waits (PENDING,
+//     gpu_driver.inl:523-528) and retries next step without resampling;
+//   * detections update per-pixel SE/BSE counters, a 1-eV histogram and the
+//     work counters used for the roofline.
+// Every wavefront exits once the primary counter is exhausted, its stack is
+// empty and all its lanes are idle; each electron is capped at 2^20 steps.
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+#include "driver.h"
+
+using namespace nbl;
+
+namespace {
+
+constexpr uint32_t STEP_CAP = 1u << 20;
+constexpr int N_COUNTERS = 14;
+// counter slots
+enum { C_PRIM = 0, C_SE, C_BSE, C_EVENTS, C_L1D, C_L2D, C_RD, C_NODES, C_TRIS, C_BOUND, C_SEC, C_PEND, C_KILLED, C_DET };
+
+struct pers_params {
+    dgeom g;
+    const dmat* mats;
+    float min_e, max_e;
+    const nbl_pixel* pixels;
+    const uint64_t* offsets;            // n_pixels + 1 prefix sums of n_primaries
+    uint32_t n_pixels;
+    uint64_t total;
+    f3 dir;
+    uint64_t prim_seed, rng_seed, rng_first;
+    unsigned long long* next;
+    uint32_t* pixel_se;
+    uint32_t* pixel_bse;
+    unsigned long long* hist;
+    unsigned long long* counters;
+    float* overflow;                    // per-wavefront HBM stack regions, 9 x ovf floats each
+    uint32_t depth;                     // LDS stack entries per wavefront
+    uint32_t ovf;                       // HBM stack entries per wavefront
+    uint32_t refill_threshold;
+};
+
+// One logical stack per wavefront: entries [0, D) in LDS, [D, D + G) in HBM.
+// Fields are stored SoA: x y z dx dy dz e mat tag.
+struct wave_stack {
+    float* lds; float* hbm; uint32_t D, G;
+    static __device__ __forceinline__ void store(float* b, size_t s, const dparticle& q, int mat, uint32_t tag)
+    {
+        b[0] = q.pos.x; b[s] = q.pos.y; b[2 * s] = q.pos.z;
+        b[3 * s] = q.dir.x; b[4 * s] = q.dir.y; b[5 * s] = q.dir.z;
+        b[6 * s] = q.e; b[7 * s] = __int_as_float(mat); b[8 * s] = __uint_as_float(tag);
+    }
+    static __device__ __forceinline__ void load(const float* b, size_t s, dparticle& q, int& mat, uint32_t& tag)
+    {
+        q.pos = { b[0], b[s], b[2 * s] };
+        q.dir = { b[3 * s], b[4 * s], b[5 * s] };
+        q.e = b[6 * s]; mat = __float_as_int(b[7 * s]); tag = __float_as_uint(b[8 * s]);
+    }
+    __device__ __forceinline__ void put(uint32_t j, const dparticle& q, int mat, uint32_t tag) const
+    {
+        if (j < D) store(lds + j, D, q, mat, tag);
+        else store(hbm + (j - D), G, q, mat, tag);
+    }
+    __device__ __forceinline__ void get(uint32_t j, dparticle& q, int& mat, uint32_t& tag) const
+    {
+        if (j < D) load(lds + j, D, q, mat, tag);
+        else load(hbm + (j - D), G, q, mat, tag);
+    }
+};
+
+__device__ __forceinline__ uint64_t bcast64(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Primary i: pixel by binary search, position from a counter-based hash.
+__device__ void make_primary(const pers_params& P, uint64_t i, dparticle& p, uint32_t& tag)
+{
+    uint32_t lo = 0, hi = P.n_pixels;                    // offsets[lo] <= i < offsets[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (P.offsets[mid] <= i) lo = mid; else hi = mid;
+    }
+    const nbl_pixel px = P.pixels[lo];
+    const uint64_t h = mix64(P.prim_seed + 0x9E3779B97F4A7C15ull * (i + 1));
+    const float u1 = (float)((uint32_t)h >> 8) * 5.9604644775390625e-8f + 2.98023223876953125e-8f;  // (0,1)
+    const float u2 = (float)((uint32_t)(h >> 32) >> 8) * 5.9604644775390625e-8f;
+    float x = px.x, y = px.y;
+    if (px.profile == 0) {
+        const float r = px.sigma * sqrtf(-2.0f * logf(u1));
+        x += r * cosf(2 * PI_F * u2);
+        y += r * sinf(2 * PI_F * u2);
+    } else {
+        x += px.half_x * (2 * u1 - 1);
+        y += px.half_y * (2 * u2 - 1);
+    }
+    p.pos = { x, y, px.z };
+    p.dir = P.dir;
+    p.e = px.energy;
+    tag = lo;
+}
+
+__global__ __launch_bounds__(1024) void k_persistent(pers_params P)
+{
+    extern __shared__ float lds[];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = lane_id();
+    const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + wave;
+    const wave_stack st{ lds + (size_t)wave * 9 * P.depth, P.overflow + (size_t)gwave * 9 * P.ovf, P.depth, P.ovf };
+    const uint32_t cap = P.depth + P.ovf;
+
+    rng_xorwow rng{ rng_seed(P.rng_seed, P.rng_first + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x) };
+    dparticle p{ { 0, 0, 0 }, { 0, 0, 1 }, 0 };
+    int mat = MAT_VACUUM;
+    uint32_t last = NO_TRI, tag = 0, steps = 0;
+    bool alive = false, pend = false;
+    uint32_t sp = 0;                                // wave-uniform stack size
+    bool queue_empty = false;                       // wave-uniform
+    lane_counters c;
+    uint32_t n_se = 0, n_bse = 0, killed = 0;
+    uint64_t prims = 0;
+
+    for (;;) {
+        // ---- refill idle lanes: stack first, then fresh primaries ----------
+        const uint64_t idle = __ballot(!alive);
+        const uint32_t n_idle = __popcll(idle);
+        if (n_idle && (n_idle >= P.refill_threshold || n_idle == 64)) {
+            const uint32_t take = min(n_idle, sp);
+            const uint32_t r = rank_below(idle);
+            if (!alive && r < take) {
+                st.get(sp - 1 - r, p, mat, tag);
+                last = NO_TRI; steps = 0; pend = false; alive = true;
+            }
+            sp -= take;
+            wave_sync();
+            const uint32_t need = n_idle - take;
+            if (need && !queue_empty) {
+                uint64_t base = 0;
+                if (lane == 0) base = atomicAdd(P.next, (unsigned long long)need);
+                base = bcast64(base);
+                if (base + need >= P.total) queue_empty = true;
+                const uint64_t fresh = __ballot(!alive);
+                if (!alive) {
+                    const uint64_t i = base + rank_below(fresh);
+                    if (i < P.total) {
+                        make_primary(P, i, p, tag);
+                        mat = MAT_VACUUM; last = NO_TRI; steps = 0; pend = false; alive = true;
+                        ++prims;
+                    }
+                }
+            }
+        }
+        if (__ballot(alive) == 0) break;
+
+        // ---- init step: sample, propagate, move ----------------------------
+        uint8_t ev = EV_NONE;
+        if (alive) {
+            if (pend) {
+                ev = EV_INELASTIC;
+            } else if (++steps > STEP_CAP) {
+                alive = false; ++killed;
+            } else {
+                ev = init_step<true>(P.g, P.mats, P.min_e, P.max_e, p, mat, last, rng, c);
+                if (ev == EV_TERMINATED) alive = false;
+            }
+        }
+        // ---- inelastic events need room for a secondary --------------------
+        // A wavefront whose live lanes all wait on a full stack goes ahead
+        // anyway and drops (counts) the secondaries that do not fit, so that
+        // every wavefront keeps making progress.
+        const bool want = alive && ev == EV_INELASTIC;
+        const uint64_t wm = __ballot(want);
+        const uint32_t room = cap - sp;
+        const bool stuck = room == 0 && wm == __ballot(alive);
+        const bool go = want && (stuck || rank_below(wm) < room);
+        if (want && !go) { pend = true; ++c.pending; }
+
+        // ---- event ----------------------------------------------------------
+        bool made = false;
+        dparticle q;
+        if (alive) {
+            if (ev == EV_INTERSECT) {
+                ++c.boundary;
+                const tri_rec t = load_tri(P.g, last);
+                const int res = intersect_execute(P.mats, t, p, mat, rng);
+                if (res == OUT_DETECTED) {
+                    alive = false;
+                    if (p.e < 50) { ++n_se; atomicAdd(&P.pixel_se[tag], 1u); }
+                    else { ++n_bse; atomicAdd(&P.pixel_bse[tag], 1u); }
+                    const int bin = (int)p.e;
+                    if (bin >= 0 && bin < NBL_HIST_BINS) atomicAdd(&P.hist[bin], 1ull);
+                } else if (res == OUT_TERMINATED) {
+                    alive = false;
+                }
+            } else if (ev == EV_ELASTIC) {
+                last = NO_TRI;
+                elastic_execute(P.mats[mat], p, rng);
+                ++c.lookups_2d;
+            } else if (go) {
+                pend = false;
+                last = NO_TRI;
+                made = inelastic_execute(P.mats[mat], p, rng, &q);
+                ++c.lookups_2d; ++c.rounddowns;
+            }
+        }
+        // ---- push secondaries ------------------------------------------------
+        const uint64_t mm = __ballot(made);
+        if (made) {
+            ++c.secondaries;
+            const uint32_t j = sp + rank_below(mm);
+            if (j < cap) st.put(j, q, mat, tag);
+            else ++killed;
+        }
+        sp = min(sp + (uint32_t)__popcll(mm), cap);
+        wave_sync();
+    }
+
+    // ---- counters: one atomic per wavefront per counter ----------------------
+    const uint64_t vals[N_COUNTERS] = { prims, n_se, n_bse, c.events, c.lookups_1d, c.lookups_2d, c.rounddowns,
+                                        c.node_steps, c.tri_tests, c.boundary, c.secondaries, c.pending, killed,
+                                        (uint64_t)(n_se + n_bse) };
+    for (int k = 0; k < N_COUNTERS; ++k) {
+        uint64_t v = vals[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0 && v) atomicAdd(&P.counters[k], (unsigned long long)v);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const nbl_persistent_opts* opts,
+    nbl_tallies* out)
+{
+    return guarded([&] {
+        require(d != nullptr, "nbl_driver_run_persistent: NULL driver");
+        require(batch && out, "nbl_driver_run_persistent: NULL batch or tallies");
+        require(batch->n_pixels > 0 && batch->pixels, "nbl_driver_run_persistent: empty pixel batch");
+        const uint32_t np = batch->n_pixels;
+        std::vector<uint64_t> offsets(np + 1, 0);
+        for (uint32_t i = 0; i < np; ++i) {
+            const nbl_pixel& px = batch->pixels[i];
+            require(px.profile <= 1, "nbl_driver_run_persistent: pixel profile must be 0 or 1");
+            require(std::isfinite(px.x) && std::isfinite(px.y) && std::isfinite(px.z) && px.energy > 0,
+                "nbl_driver_run_persistent: pixel position/energy invalid");
+            offsets[i + 1] = offsets[i] + px.n_primaries;
+        }
+        const uint64_t total = offsets[np];
+        const f3 dir{ batch->dir.x, batch->dir.y, batch->dir.z };
+        const float dn = std::sqrt(dir.x * dir.x + dir.y * dir.y + dir.z * dir.z);
+        require(dn > 0 && std::isfinite(dn), "nbl_driver_run_persistent: beam direction is zero");
+
+        int dev = 0, cus = 0;
+        NBL_HIP(hipGetDevice(&dev));
+        NBL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        uint32_t block = opts && opts->block_size ? opts->block_size : 256;
+        uint32_t per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : 4;
+        uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 128;
+        uint32_t thresh = opts && opts->refill_threshold ? opts->refill_threshold : 1;
+        uint32_t ovf = opts && opts->overflow_depth ? opts->overflow_depth : 4096;
+        require(ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
+        require(block % 64 == 0 && block >= 64 && block <= 1024, "nbl_driver_run_persistent: block_size must be 64..1024, multiple of 64");
+        require(per_cu >= 1 && per_cu <= 32, "nbl_driver_run_persistent: blocks_per_cu must be 1..32");
+        require(depth >= 64 && depth <= 1024, "nbl_driver_run_persistent: stack_depth must be 64..1024");
+        require(thresh >= 1 && thresh <= 64, "nbl_driver_run_persistent: refill_threshold must be 1..64");
+        const size_t lds = (size_t)(block / 64) * 9 * depth * sizeof(float);
+        require(lds <= 160 * 1024, "nbl_driver_run_persistent: stack_depth x waves exceeds the LDS size");
+
+        dbuf<nbl_pixel> dpix(np);
+        dbuf<uint64_t> doff(np + 1);
+        dbuf<unsigned long long> dnext(1), dhist(NBL_HIST_BINS), dcnt(N_COUNTERS);
+        dbuf<uint32_t> dse(np), dbse(np);
+        hipStream_t s = d->stream;
+        dpix.upload(batch->pixels, np, s);
+        doff.upload(offsets.data(), np + 1, s);
+        NBL_HIP(hipMemsetAsync(dnext.p, 0, sizeof(unsigned long long), s));
+        NBL_HIP(hipMemsetAsync(dhist.p, 0, NBL_HIST_BINS * sizeof(unsigned long long), s));
+        NBL_HIP(hipMemsetAsync(dcnt.p, 0, N_COUNTERS * sizeof(unsigned long long), s));
+        NBL_HIP(hipMemsetAsync(dse.p, 0, np * sizeof(uint32_t), s));
+        NBL_HIP(hipMemsetAsync(dbse.p, 0, np * sizeof(uint32_t), s));
+
+        pers_params P;
+        P.g = geometry_view(d->geom);
+        P.mats = materials_device(d->mats);
+        P.min_e = d->min_energy; P.max_e = d->max_energy;
+        P.pixels = dpix.p; P.offsets = doff.p; P.n_pixels = np; P.total = total;
+        P.dir = { dir.x / dn, dir.y / dn, dir.z / dn };
+        P.prim_seed = batch->seed;
+        P.rng_seed = d->seed;
+        const uint64_t threads = (uint64_t)cus * per_cu * block;
+        P.rng_first = (uint64_t)d->cap + (d->runs++) * threads;   // streams distinct from the driver's slots
+        P.next = dnext.p; P.pixel_se = dse.p; P.pixel_bse = dbse.p; P.hist = dhist.p; P.counters = dcnt.p;
+        const size_t n_waves = (size_t)cus * per_cu * (block / 64);
+        const size_t ovf_floats = n_waves * 9 * ovf;
+        if (d->overflow.n < ovf_floats) d->overflow.alloc(ovf_floats);
+        P.overflow = d->overflow.p;
+        P.depth = depth; P.ovf = ovf; P.refill_threshold = thresh;
+
+        hipEvent_t e0, e1;
+        NBL_HIP(hipEventCreate(&e0)); NBL_HIP(hipEventCreate(&e1));
+        NBL_HIP(hipEventRecord(e0, s));
+        hipLaunchKernelGGL(k_persistent, dim3((unsigned)(cus * per_cu)), dim3(block), lds, s, P);
+        NBL_HIP(hipGetLastError());
+        NBL_HIP(hipEventRecord(e1, s));
+        NBL_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        NBL_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+
+        uint64_t cnt[N_COUNTERS];
+        dcnt.download(reinterpret_cast<unsigned long long*>(cnt), N_COUNTERS, s);
+        std::vector<unsigned long long> hist(NBL_HIST_BINS);
+        dhist.download(hist.data(), NBL_HIST_BINS, s);
+        if (out->pixel_se) dse.download(out->pixel_se, np, s);
+        if (out->pixel_bse) dbse.download(out->pixel_bse, np, s);
+        NBL_HIP(hipStreamSynchronize(s));
+        if (out->hist) for (int b = 0; b < NBL_HIST_BINS; ++b) out->hist[b] = (uint32_t)hist[b];
+        out->primaries = cnt[C_PRIM]; out->detected_se = cnt[C_SE]; out->detected_bse = cnt[C_BSE];
+        out->events = cnt[C_EVENTS]; out->lookups_1d = cnt[C_L1D]; out->lookups_2d = cnt[C_L2D];
+        out->rounddowns = cnt[C_RD]; out->node_steps = cnt[C_NODES]; out->tri_tests = cnt[C_TRIS];
+        out->boundary_events = cnt[C_BOUND]; out->secondaries = cnt[C_SEC]; out->pending_retries = cnt[C_PEND];
+        out->stopped = cnt[C_KILLED];
+        out->kernel_ms = ms;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+"""GPU tests of the driver: in-house status sort, XORWOW streams, one
+do_iteration at a time against the oracle's GPU-driver semantics, the
+buffered frame loop of gpu_main.cu, and full-run statistics (yields within
+3 sigma, chi-square and KS on detected-energy spectra) of the iteration
+driver and the persistent kernel against the oracle's CPU driver."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import runs
+import nebula_amd as na
+from nebula_amd import _lib as L
+from nebula_amd import scenes, synth
+from nebula_amd.driver import Driver, WorkPool, raster_scan
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [1, 63, 4096, 4097, 1_000_003])
+def test_status_sort_is_stable_counting_sort(n):
+    print(f"[gpu] sort n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    codes = np.array([L.ST_PENDING, L.ST_INELASTIC, L.ST_ELASTIC, L.ST_INTERSECT, L.ST_NO_EVENT,
+                      L.ST_DETECTED, L.ST_NEW_SECONDARY, L.ST_TERMINATED], np.uint8)
+    keys = rng.choice(codes, n).astype(np.uint8)
+    out = np.zeros(n, np.uint32)
+    L.call("nbl_sort_status_batch", L.ptr(keys), n, L.ptr(out))
+    assert np.array_equal(out, np.argsort(keys & 3, kind="stable").astype(np.uint32))
+    same = np.full(n, L.ST_TERMINATED, np.uint8)
+    L.call("nbl_sort_status_batch", L.ptr(same), n, L.ptr(out))
+    assert np.array_equal(out, np.arange(n, dtype=np.uint32))
+
+
+def test_xorwow_streams_match_oracle():
+    print("[gpu] rng", flush=True)
+    n, draws = 10_000, 64
+    a = np.zeros(6 * n, np.uint32); b = np.zeros(6 * n, np.uint32)
+    L.call("nbl_rng_seed", 1234, 77, n, L.ptr(a))
+    ol.orc.orc_rng_seed(1234, 77, n, ol.p(b))
+    assert np.array_equal(a, b)
+    ga = np.zeros(n * draws, np.float32); oa = np.zeros(n * draws, np.float32)
+    L.call("nbl_rng_uniform_batch", L.ptr(a), n, draws, L.ptr(ga))
+    ol.orc.orc_rng_uniform_batch(ol.p(b), n, draws, ol.p(oa))
+    assert np.array_equal(ga, oa)
+
+
+def _scene_lines():
+    tris = scenes.line_trench()
+    specs = [synth.SI, synth.PMMA]
+    return tris, specs
+
+
+def test_iterations_match_oracle_stepwise():
+    """do_iteration on the GPU vs the oracle's restatement of
+    gpu_driver.inl:312-567 from the same slot state, one iteration at a time
+    (the oracle is re-synchronised to the GPU state after every step).
+    Capacity is small so that secondaries run out of free slots and the
+    PENDING path is taken."""
+    print("[gpu] stepwise iterations", flush=True)
+    tris, specs = _scene_lines()
+    g = na.Geometry(tris); m = na.Materials(specs)
+    og = ol.OrcGeometry(tris); om = ol.OrcMaterials([p.desc for p in m.prepared])
+    cap = 3000
+    drv = Driver(g, m, capacity=cap, seed=42)
+    osl = ol.orc.orc_slots_create(cap, 42)
+    try:
+        rng = np.random.default_rng(1)
+        n = 2000
+        prim = np.zeros(n, L.PARTICLE_DTYPE)
+        prim["pos"] = np.stack([rng.uniform(-200, 200, n), rng.uniform(-200, 200, n), np.full(n, 55.0)], 1)
+        prim["dir"] = (0, 0, -1)
+        prim["kin_energy"] = rng.uniform(100, 1500, n)
+        tags = np.arange(n, dtype=np.uint32)
+        assert drv.push(prim, tags) == n
+        assert ol.orc.orc_slots_push(osl, ol.p(prim), ol.p(tags), n) == n
+        max_e = m.max_energy()
+        seen_pending = False
+        for it in range(40):
+            s0 = drv.read_slots()
+            view = L.SlotView(**{k: ol.p(v) for k, v in s0.items()})
+            ol.orc.orc_slots_write(osl, C.byref(view))
+            drv.do_iteration()
+            ol.orc.orc_slots_iteration(osl, om.h, og.h, 0.0, max_e)
+            a = drv.read_slots()
+            b = {k: np.zeros_like(v) for k, v in s0.items()}
+            ol.orc.orc_slots_read(osl, C.byref(L.SlotView(**{k: ol.p(v) for k, v in b.items()})))
+            assert np.array_equal(a["particle_idx"], b["particle_idx"]), f"iteration {it}: sort differs"
+            st_eq = (a["status"] == b["status"]).mean()
+            assert st_eq > 0.995, f"iteration {it}: status agreement {st_eq}"
+            for k in ("tags", "material", "last_tri"):
+                assert (a[k] == b[k]).mean() > 0.995, f"iteration {it}: {k}"
+            active = (a["status"] != L.ST_TERMINATED) & (b["status"] != L.ST_TERMINATED)
+            pa, pb = a["particles"][active], b["particles"][active]
+            ok = np.logical_and.reduce([np.all(np.isclose(pa[f], pb[f], rtol=1e-4, atol=1e-4).reshape(len(pa), -1), axis=1)
+                                        for f in ("pos", "dir", "kin_energy")])
+            assert ok.mean() > 0.99, f"iteration {it}: particle agreement {ok.mean()}"
+            # an event's RNG use is identical: the per-slot streams stay in lockstep
+            assert (a["rng"].reshape(-1, 6) == b["rng"].reshape(-1, 6)).all(axis=1).mean() > 0.995
+            seen_pending |= bool((a["status"] == L.ST_PENDING).any())
+        assert seen_pending, "the PENDING path was not exercised"
+    finally:
+        ol.orc.orc_slots_destroy(osl)
+        drv.close()
+
+
+def test_buffered_frame_loop_runs_all_primaries():
+    """gpu_main.cu:178-213: input buffer refills, detection snapshot, frames."""
+    print("[gpu] frame loop", flush=True)
+    g = na.Geometry(scenes.cube_slab()); m = na.Materials([synth.SI])
+    n = 3000
+    pool = WorkPool(runs.pencil_primaries(n, 1000.0), np.arange(n, dtype=np.uint32))
+    detected = []
+    with Driver(g, m, capacity=200_000, seed=7) as drv:
+        drv.allocate_input_buffers(1500)
+        drv.push_to_buffer(pool)
+        for frame in range(100_000):
+            drv.push_to_simulation()
+            drv.buffer_detected()
+            for _ in range(20):
+                drv.do_iteration()
+            drv.push_to_buffer(pool)
+            running, det = drv.flush_buffered()
+            detected += det
+            if running == 0 and pool.done():
+                break
+        drv.buffer_detected()
+        running, det = drv.flush_buffered()
+        detected += det
+        assert running == 0 and pool.done()
+    tags = np.array([t for _, t in detected])
+    assert tags.max() < n
+    e = np.array([p["kin_energy"] for p, _ in detected])
+    # BSE yield of 1 keV on the synthetic Si: compare with the oracle
+    og = ol.OrcGeometry(scenes.cube_slab()); om = ol.OrcMaterials([m.prepared[0].desc])
+    st, _ = runs.oracle_run(og, om, runs.pencil_primaries(n, 1000.0), threads=8)
+    z = runs.yield_z((e >= 50).sum(), n, st["detected_bse"], st["primaries"])
+    assert abs(z) < 4, z
+
+
+def _per_primary_counts(det_tags, n):
+    return np.bincount(det_tags, minlength=n)
+
+
+@pytest.mark.parametrize("energy", [1000.0, 300.0])
+def test_persistent_yields_and_spectra_match_oracle(energy):
+    """Cube slab, pencil beam: SE/BSE yields within 3 sigma, chi-square and KS
+    on the detected-energy histogram, persistent kernel vs oracle CPU driver."""
+    print(f"[gpu] persistent yields {energy} eV", flush=True)
+    tris = scenes.cube_slab()
+    g = na.Geometry(tris); m = na.Materials([synth.SI])
+    og = ol.OrcGeometry(tris); om = ol.OrcMaterials([m.prepared[0].desc])
+    n_cpu = 12_000 if energy > 500 else 30_000
+    st, h_cpu = runs.oracle_run(og, om, runs.pencil_primaries(n_cpu, energy), threads=16)
+    # GPU: 400 pixels at the beam point, so per-pixel yields give the variance
+    n_pix, per = 400, 500
+    beam = raster_scan(np.zeros(20), np.zeros(20), 5.0, energy, per, 0.0, seed=3)
+    with Driver(g, m, capacity=1 << 16, seed=11) as drv:
+        t = drv.run_persistent(beam, per_pixel=True, hist=True)
+    n_gpu = n_pix * per
+    assert t["primaries"] == n_gpu and t["stopped"] == 0
+    for key, pix in (("detected_se", "pixel_se"), ("detected_bse", "pixel_bse")):
+        y_gpu = t[pix].astype(float) / per
+        se_gpu = y_gpu.std(ddof=1) / np.sqrt(n_pix)
+        y_cpu = st[key] / st["primaries"]
+        # CPU standard error from the GPU's per-primary dispersion (same physics)
+        se_cpu = y_gpu.std(ddof=1) * np.sqrt(per) / np.sqrt(st["primaries"])
+        z = (y_gpu.mean() - y_cpu) / np.hypot(se_gpu, se_cpu)
+        print(f"  {key}: gpu {y_gpu.mean():.4f} cpu {y_cpu:.4f} z {z:+.2f}")
+        assert abs(z) < 3, (key, z)
+    p, chi, dof = runs.chi2_hist(t["hist"], h_cpu)
+    d, pks = runs.ks_binned(t["hist"], h_cpu)
+    print(f"  spectrum chi2 p={p:.3g} (chi {chi:.1f}/{dof}), KS D={d:.4f} p={pks:.3g}")
+    assert p > 1e-3 and pks > 1e-3
+
+
+def test_iteration_driver_and_persistent_agree():
+    """The two GPU transport forms (reference-semantics iterations and the
+    persistent kernel) give the same yields within 3 sigma."""
+    print("[gpu] iteration vs persistent", flush=True)
+    tris = scenes.cube_slab()
+    g = na.Geometry(tris); m = na.Materials([synth.SI])
+    n = 20_000
+    with Driver(g, m, capacity=1 << 20, seed=5) as drv:
+        assert drv.push(runs.pencil_primaries(n, 1000.0), np.arange(n, dtype=np.uint32)) == n
+        det = []
+        for _ in range(100_000):
+            drv.do_iteration()
+            if drv.get_running_count() == 0:
+                break
+        det = drv.flush_detected()
+        assert drv.get_running_count() == 0
+    e = np.array([p["kin_energy"] for p, _ in det])
+    with Driver(g, m, capacity=1 << 16, seed=6) as drv:
+        t = drv.run_persistent(raster_scan(np.zeros(1), np.zeros(1), 5.0, 1000.0, 200_000, 0.0, seed=9))
+    for k_it, k_p in (((e < 50).sum(), t["detected_se"]), ((e >= 50).sum(), t["detected_bse"])):
+        z = runs.yield_z(k_it, n, k_p, t["primaries"])
+        print(f"  iteration {k_it / n:.4f} persistent {k_p / t['primaries']:.4f} z {z:+.2f}")
+        assert abs(z) < 4

@@ -1,0 +1,216 @@
+"""Pins of the CPU oracle (CPU only).
+
+Each oracle function is checked against something that fixes it
+independently of its own formula: the reference's own code (golden fixtures
+written by tests/golden/make_golden.py from oracle/_ref, and the live harness
+where it is built), brute force, closed forms, or invariants.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from nebula_amd import scenes
+from nebula_amd._lib import NO_TRIANGLE, TRIANGLE_DTYPE
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+GEOMS = ["cube", "soup", "lines"]
+
+
+def load(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+@pytest.mark.parametrize("name", GEOMS)
+def test_builder_matches_reference_fixture(name):
+    f = load(f"geometry_{name}.npz")
+    g = ol.OrcGeometry(f["triangles"])
+    d, o = g.export()
+    assert np.array_equal(d, f["octree_data"])
+    assert np.array_equal(o, f["triangle_order"])
+    i = g.info()
+    assert np.array_equal(i["center"], f["center"]) and np.array_equal(i["halfsize"], f["halfsize"])
+    assert np.float32(i["max_extent"]) == f["max_extent"]
+
+
+@pytest.mark.parametrize("name", GEOMS)
+def test_propagate_matches_reference_fixture(name):
+    f = load(f"geometry_{name}.npz")
+    g = ol.OrcGeometry(f["triangles"])
+    od, ot = g.propagate(f["pos"], f["dir"], f["dist"], f["ignore_tri"], f["ignore_mat"])
+    assert np.array_equal(ot, f["out_tri"])
+    assert np.array_equal(od.view(np.uint32), f["out_dist"].view(np.uint32))
+    assert np.array_equal(g.in_domain(f["pos"]), f["in_domain"])
+
+
+def test_tables_and_intersect_match_reference_fixture():
+    from nebula_amd._lib import MaterialDesc, Table1D, Table2D
+    import ctypes as C
+    f = load("tables.npz")
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    d1, d2 = f["d1"], np.ascontiguousarray(f["d2"])
+    t1, t2 = f["t1"], f["t2"]
+    desc = MaterialDesc()
+    desc.elastic_log_imfp = Table1D(len(d1), t1[0], t1[1], fp(d1))
+    desc.inelastic_log_imfp = Table1D(len(d1), t1[0], t1[1], fp(d1))
+    for k in ("elastic_costheta_icdf", "inelastic_log_w0_icdf", "ionisation"):
+        setattr(desc, k, Table2D(d2.shape[0], d2.shape[1], t2[0], t2[1], t2[2], t2[3], fp(d2)))
+    m = ol.OrcMaterials([desc])
+    x1, x2, y2 = f["x1"], f["x2"], f["y2"]
+    o1 = np.zeros(len(x1), np.float32)
+    ol.orc.orc_table1d_get_batch(m.h, 0, 0, ol.p(x1), len(o1), ol.p(o1))
+    assert np.array_equal(o1.view(np.uint32), f["o1"].view(np.uint32))
+    for rd, key in ((0, "o2"), (1, "o3")):
+        o = np.zeros(len(x2), np.float32)
+        ol.orc.orc_table2d_get_batch(m.h, 0, 2, rd, ol.p(x2), ol.p(y2), len(o), ol.p(o))
+        assert np.array_equal(o.view(np.uint32), f[key].view(np.uint32))
+    tri, tpos, tdir = f["tri"], f["tri_pos"], f["tri_dir"]
+    t = np.zeros(len(tri), np.float32)
+    ol.orc.orc_intersect_ray_batch(ol.p(tri), ol.p(tpos), ol.p(tdir), len(t), ol.p(t))
+    assert np.array_equal(t.view(np.uint32), f["tri_t"].view(np.uint32))
+
+
+@pytest.mark.skipif(ol.ref is None, reason="reference harness not built (no /root/reference)")
+def test_rough_surface_against_reference_code():
+    """Full-size rough surface (526k triangles): linearization and 100k rays."""
+    tris = scenes.rough_surface()
+    g, r = ol.OrcGeometry(tris), ol.RefGeometry(tris)
+    gd, go = g.export(); rd, ro = r.export()
+    assert np.array_equal(gd, rd) and np.array_equal(go, ro)
+    rng = np.random.default_rng(3)
+    n = 100_000
+    pos = np.stack([rng.uniform(-520, 520, n), rng.uniform(-520, 520, n), rng.uniform(-30, 25, n)], 1).astype(np.float32)
+    d = rng.standard_normal((n, 3)); d = (d / np.linalg.norm(d, axis=1)[:, None]).astype(np.float32)
+    dist = rng.exponential(20, n).astype(np.float32)
+    it = np.where(rng.random(n) < 0.3, rng.integers(0, len(tris), n), NO_TRIANGLE).astype(np.uint32)
+    im = rng.choice([0, -123], n).astype(np.int32)
+    a = g.propagate(pos, d, dist, it, im); b = r.propagate(pos, d, dist, it, im)
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+
+
+def test_propagate_agrees_with_brute_force():
+    """octree::propagate vs trilist::propagate (geometry/trilist.inl:52-85):
+    same triangle except where two hits lie within EPSILON (tie rule differs)."""
+    f = load("geometry_soup.npz")
+    tris = f["triangles"]
+    g = ol.OrcGeometry(tris)
+    pos, d, dist, it, im = f["pos"], f["dir"], f["dist"], f["ignore_tri"], f["ignore_mat"]
+    od, ot = g.propagate(pos, d, dist, it, im)
+    bd = np.zeros(len(pos), np.float32); bt = np.zeros(len(pos), np.uint32)
+    ol.orc.orc_trilist_propagate_batch(ol.p(tris), len(tris), ol.p(pos), ol.p(d), ol.p(dist), ol.p(it), ol.p(im),
+                                       len(pos), ol.p(bd), ol.p(bt))
+    # the builder drops duplicate triangles, the brute force does not
+    _, order = g.export()
+    dropped = np.setdiff1d(np.arange(len(tris)), order)
+    assert len(dropped) == 2
+    keep = ~np.isin(bt, dropped) & ~np.isin(it, dropped)
+    ot, bt, od, bd = ot[keep], bt[keep], od[keep], bd[keep]
+    same = ot == bt
+    # disagreements only when both hit, at distances within a few EPSILON
+    both = (ot != NO_TRIANGLE) & (bt != NO_TRIANGLE) & ~same
+    assert np.all(np.abs(od[both] - bd[both]) < 1e-3)
+    assert same.mean() > 0.99
+    hit = same & (ot != NO_TRIANGLE)
+    assert np.allclose(od[hit], bd[hit], rtol=1e-5, atol=1e-4)
+
+
+def _tri(r0, r1, r2, mi=0, mo=-123):
+    t = np.zeros(1, TRIANGLE_DTYPE)
+    t["r0"] = r0; t["e1"] = np.subtract(r1, r0); t["e2"] = np.subtract(r2, r0)
+    t["material_in"] = mi; t["material_out"] = mo
+    return t
+
+
+def test_intersect_ray_closed_forms():
+    """Axis-aligned rays against a unit triangle give exact t; misses, parallel
+    rays (det < EPSILON) and hits behind the start give -1."""
+    t = _tri((0, 0, 0), (1, 0, 0), (0, 1, 0))
+    cases = [((0.25, 0.25, 3.0), (0, 0, -1), 3.0), ((0.25, 0.25, -2.0), (0, 0, 1), 2.0),
+             ((0.9, 0.9, 1.0), (0, 0, -1), -1.0), ((0.2, 0.2, 1.0), (1, 0, 0), -1.0),
+             ((0.2, 0.2, -1.0), (0, 0, -1), -1.0), ((1.0, 0.0, 5.0), (0, 0, -1), 5.0),
+             ((0.5, 0.5, 0.125), (0, 0, -1), 0.125)]
+    tris = np.repeat(t, len(cases))
+    pos = np.array([c[0] for c in cases], np.float32); d = np.array([c[1] for c in cases], np.float32)
+    out = np.zeros(len(cases), np.float32)
+    ol.orc.orc_intersect_ray_batch(ol.p(tris), ol.p(pos), ol.p(d), len(cases), ol.p(out))
+    assert np.array_equal(out, np.array([c[2] for c in cases], np.float32))
+
+
+def test_single_leaf_reduces_to_brute_force_plus_cell_exit():
+    """With <= 16 triangles the octree has one leaf per octant, and a ray that
+    stays in the domain sees the brute-force nearest hit."""
+    rng = np.random.default_rng(4)
+    tris = np.concatenate([_tri(rng.uniform(-5, 5, 3), rng.uniform(-5, 5, 3), rng.uniform(-5, 5, 3)) for _ in range(10)])
+    g = ol.OrcGeometry(tris)
+    n = 3000
+    pos = rng.uniform(-3, 3, (n, 3)).astype(np.float32)
+    d = rng.standard_normal((n, 3)); d = (d / np.linalg.norm(d, axis=1)[:, None]).astype(np.float32)
+    dist = np.full(n, 2.0, np.float32)
+    it = np.full(n, NO_TRIANGLE, np.uint32); im = np.full(n, 12345, np.int32)
+    od, ot = g.propagate(pos, d, dist, it, im)
+    bd = np.zeros(n, np.float32); bt = np.zeros(n, np.uint32)
+    ol.orc.orc_trilist_propagate_batch(ol.p(tris), len(tris), ol.p(pos), ol.p(d), ol.p(dist), ol.p(it), ol.p(im), n, ol.p(bd), ol.p(bt))
+    assert (ot == bt).mean() > 0.999
+    # invariants: distance <= requested; a hit lies on the returned triangle
+    assert np.all(od <= dist)
+
+
+def test_builder_invariants_on_lines():
+    tris = scenes.line_trench()
+    g = ol.OrcGeometry(tris)
+    d, order = g.export()
+    i = g.info()
+    # every triangle (no duplicates in this scene) reaches a leaf
+    assert i["n_unique"] == len(tris) and np.array_equal(np.sort(order), np.arange(len(tris)))
+    # leaves hold at most 16 triangles unless at depth 21
+    leaf_sizes, cur = [], 0
+    assert i["max_depth"] < 21
+    node = d[:8]
+    # walk: internal node entries > 0 point to nodes, < 0 to leaves
+    seen = set()
+    stack = [0]
+    while stack:
+        off = stack.pop()
+        for c in d[off:off + 8]:
+            c = int(c)
+            if c > 0:
+                stack.append(c)
+            else:
+                k = -c; n = 0
+                while d[k + n] >= 0:
+                    n += 1
+                leaf_sizes.append(n)
+                seen.update(int(x) for x in d[k:k + n])
+    assert max(leaf_sizes) <= 16
+    assert seen == set(range(len(tris)))
+
+
+def test_xorwow_recurrence_and_uniformity():
+    """Stream = Marsaglia's xorwow recurrence (checked word by word in Python
+    on an explicit state) mapped to (0,1]; and uniform by KS over 2e5 draws."""
+    from scipy import stats
+    st = np.array([123456789, 362436069, 521288629, 88675123, 5783321, 6615241], np.uint32)
+    s = st.copy()
+    out = np.zeros(8, np.float32)
+    ol.orc.orc_rng_uniform_batch(ol.p(s), 1, 8, ol.p(out))
+    x, y, z, w, v, dd = (int(a) for a in st)
+    M = 0xFFFFFFFF
+    exp = []
+    for _ in range(8):
+        t = (x ^ (x >> 2)) & M
+        x, y, z, w = y, z, w, v
+        v = ((v ^ ((v << 4) & M)) ^ (t ^ ((t << 1) & M))) & M
+        dd = (dd + 362437) & M
+        r = (dd + v) & M
+        exp.append(((r >> 8) + 1) / 16777216.0)
+    assert np.array_equal(out, np.array(exp, np.float32))
+    states = np.zeros(6 * 1000, np.uint32)
+    ol.orc.orc_rng_seed(99, 0, 1000, ol.p(states))
+    u = np.zeros(1000 * 200, np.float32)
+    ol.orc.orc_rng_uniform_batch(ol.p(states), 1000, 200, ol.p(u))
+    assert u.min() > 0 and u.max() <= 1
+    assert stats.kstest(u, "uniform").pvalue > 1e-4
+    # neighbouring sequences are uncorrelated
+    a = u.reshape(1000, 200)
+    assert abs(np.corrcoef(a[:-1].ravel(), a[1:].ravel())[0, 1]) < 0.01
