@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--stack-depth", type=int, default=0)
     ap.add_argument("--refill-threshold", type=int, default=0)
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30,
+                    help="size of the plain copy kernel run first (counter scale, bandwidth yardstick); 0 = skip")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the transport shards by primaries with no "
@@ -90,6 +92,15 @@ def main():
     name, cus = na.device_info()
     print(f"setup: {len(tris)} triangles, {info['n_nodes']} nodes, {info['n_leaves']} leaves, "
           f"device {name} ({cus} CUs), {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    copy_gbps = None
+    if args.copy_bytes:
+        import ctypes as C
+        ms = C.c_double()
+        na._lib.call("nbl_bench_copy", args.copy_bytes, 3, C.byref(ms))
+        copy_gbps = 2 * args.copy_bytes / (ms.value * 1e-3) / 1e9
+        print(f"copy kernel: {args.copy_bytes} B in {ms.value:.3f} ms = {copy_gbps:.0f} GB/s (read + write)",
+              file=sys.stderr, flush=True)
 
     opts = dict(block_size=args.block_size, blocks_per_cu=args.blocks_per_cu,
                 stack_depth=args.stack_depth, refill_threshold=args.refill_threshold)
@@ -151,7 +162,7 @@ def main():
                    "sampler": "table", "kernel": "persistent"},
         "roofline": {"bound": "hbm", "achieved": ach_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": ach_gbps / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "k_persistent", "avg_kernel_ms": avg_kernel_ms,
+                     "kernel": "k_persistent", "avg_kernel_ms": avg_kernel_ms, "copy_kernel_GBps": copy_gbps,
                      "algorithmic_bytes_per_launch": algorithmic_bytes(per_launch)},
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,

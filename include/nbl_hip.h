@@ -111,6 +111,10 @@ const char* nbl_last_error(void);
 int nbl_version(int* major, int* minor, int* patch);
 /* Name of device 0 and its compute-unit count. */
 int nbl_device_info(char* name, size_t name_len, int* compute_units);
+/* No reference counterpart: a plain device-to-device copy kernel of `bytes`
+ * (multiple of 16) run `reps` times, for scaling hardware byte counters and
+ * as an achievable-bandwidth yardstick; *ms_per_copy = mean duration. */
+int nbl_bench_copy(uint64_t bytes, int32_t reps, double* ms_per_copy);
 
 /* ---- geometry: replaces geometry::octree<true> (geometry/octree.h:31-111) --- */
 typedef struct nbl_geometry_s* nbl_geometry;

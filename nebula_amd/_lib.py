@@ -137,6 +137,7 @@ _U32, _I32, _U64, _F = C.c_uint32, C.c_int32, C.c_uint64, C.c_float
 SIGNATURES = {
     "nbl_version": [C.POINTER(C.c_int)] * 3,
     "nbl_device_info": [C.c_char_p, C.c_size_t, C.POINTER(C.c_int)],
+    "nbl_bench_copy": [_U64, _I32, C.POINTER(C.c_double)],
     "nbl_geometry_create": [_P, _U32, C.POINTER(_P)],
     "nbl_geometry_destroy": [_P],
     "nbl_geometry_get_info": [_P, C.POINTER(GeometryInfo)],

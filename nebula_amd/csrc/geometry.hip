@@ -44,15 +44,18 @@ int geometry_max_material(nbl_geometry g)
 
 namespace {
 
-// One ray per thread, grid-stride.  Inputs and outputs are device arrays.
+// One ray per thread, grid-stride, with the same per-lane LDS path memo as
+// the transport kernel.  Inputs and outputs are device arrays.
 __global__ __launch_bounds__(256) void k_propagate(dgeom g, const f3* __restrict__ pos,
     const f3* __restrict__ dir, const float* __restrict__ dist, const uint32_t* __restrict__ ignore_tri,
     const int32_t* __restrict__ ignore_mat, uint32_t n, float* __restrict__ out_dist,
     uint32_t* __restrict__ out_tri)
 {
+    __shared__ uint32_t memo[4 * lds_path_cache::kLevels * 64];
+    lds_path_cache pc{ memo + (threadIdx.x >> 6) * lds_path_cache::kLevels * 64 + (threadIdx.x & 63), 1ull, 0ull, 0u, false };
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t hit;
-        out_dist[i] = propagate<false>(g, pos[i], dir[i], dist[i], ignore_tri[i], ignore_mat[i], &hit, nullptr);
+        out_dist[i] = propagate<false>(g, pos[i], dir[i], dist[i], ignore_tri[i], ignore_mat[i], &hit, nullptr, &pc);
         out_tri[i] = hit;
     }
 }
@@ -68,7 +71,8 @@ void launch_propagate(nbl_geometry g, const f3* pos, const f3* dir, const float*
     const uint32_t* it, const int32_t* im, uint32_t n, float* od, uint32_t* ot, hipStream_t s)
 {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_propagate, dim3(grid_for(n, 256)), dim3(256), 0, s, g->view, pos, dir, dist, it, im, n, od, ot);
+    // at most 4 resident blocks per CU-sized slice: lanes walk several rays each
+    hipLaunchKernelGGL(k_propagate, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, g->view, pos, dir, dist, it, im, n, od, ot);
     NBL_HIP(hipGetLastError());
 }
 

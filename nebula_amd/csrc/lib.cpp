@@ -17,7 +17,39 @@ void ensure_device()
 
 }  // namespace nbl
 
+namespace {
+// Plain streaming copy, 16 bytes per lane per iteration.
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace
+
 extern "C" {
+
+int nbl_bench_copy(uint64_t bytes, int32_t reps, double* ms_per_copy)
+{
+    return nbl::guarded([&] {
+        nbl::require(bytes >= 16 && bytes % 16 == 0 && reps >= 1, "nbl_bench_copy: bytes must be a multiple of 16, reps >= 1");
+        nbl::ensure_device();
+        const size_t n = bytes / 16;
+        nbl::dbuf<uint4> a(n), b(n);
+        NBL_HIP(hipMemset(a.p, 1, bytes));
+        hipEvent_t e0, e1;
+        NBL_HIP(hipEventCreate(&e0)); NBL_HIP(hipEventCreate(&e1));
+        NBL_HIP(hipEventRecord(e0, 0));
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_copy, dim3(256 * 8 * 4), dim3(256), 0, 0, a.p, b.p, n);
+        NBL_HIP(hipGetLastError());
+        NBL_HIP(hipEventRecord(e1, 0));
+        NBL_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        NBL_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        if (ms_per_copy) *ms_per_copy = ms / reps;
+    });
+}
 
 const char* nbl_last_error(void) { return nbl::g_last_error.c_str(); }
 

@@ -92,14 +92,50 @@ __device__ __forceinline__ float intersect_ray(f3 r0, f3 e1, f3 e2, f3 start, f3
     return t;
 }
 
+// Memo of the last descended path, so that the walk's re-descent from the
+// root (octree.inl:62-69 re-reads every level per cell) reloads only the
+// levels where the new path leaves the old one.  The AABB arithmetic is
+// always redone from the root, so results are identical with or without it.
+struct no_path_cache {
+    __device__ __forceinline__ void begin() {}
+    __device__ __forceinline__ bool hit(uint32_t, unsigned, uint64_t&) const { return false; }
+    __device__ __forceinline__ void miss(uint32_t, uint64_t) {}
+    __device__ __forceinline__ void done(uint64_t, uint32_t, uint64_t) {}
+};
+struct lds_path_cache {
+    static constexpr uint32_t kLevels = 16;   // internal nodes of levels 1..15 kept in LDS
+    uint32_t* node;      // this lane's column: node[l * 64] = internal node at depth l
+    uint64_t loc;        // location code of the cached leaf (1 = nothing cached)
+    uint64_t leaf;       // its child word
+    uint32_t depth;      // its depth
+    bool matched;        // the path descended so far equals the cached one
+    __device__ __forceinline__ void begin() { matched = true; }
+    // child word of depth l reached through octant oct, if the cache has it
+    __device__ __forceinline__ bool hit(uint32_t l, unsigned oct, uint64_t& child)
+    {
+        if (!matched || l > depth || ((unsigned)(loc >> (3 * (depth - l))) & 7u) != oct) { matched = false; return false; }
+        if (l == depth) { child = leaf; return true; }
+        if (l < kLevels) { child = node[l * 64]; return true; }
+        return false;                        // level not stored: load it, path still matches
+    }
+    __device__ __forceinline__ void miss(uint32_t l, uint64_t child)
+    {
+        if (l < kLevels && !(child & LEAF_BIT)) node[l * 64] = (uint32_t)child;
+    }
+    __device__ __forceinline__ void done(uint64_t location, uint32_t d, uint64_t child)
+    {
+        loc = location; depth = d; leaf = child;
+    }
+};
+
 // octree::propagate, geometry/octree.inl:32-192.  Same cell walk (location
 // codes, re-descent from the root, neighbour by bit flip), same AABB and
 // intersection arithmetic and the same "t <= best + EPSILON" replacement
 // rule over each leaf's triangles in leaf order.  Returns the travelled
 // distance; *hit gets the triangle's input index or NO_TRI.
-template<bool COUNT>
+template<bool COUNT, typename CACHE = no_path_cache>
 __device__ __forceinline__ float propagate(const dgeom& g, f3 start, f3 dir, float distance,
-    uint32_t ignore_tri, int ignore_mat, uint32_t* hit, prop_counters* cnt)
+    uint32_t ignore_tri, int ignore_mat, uint32_t* hit, prop_counters* cnt, CACHE* pc = nullptr)
 {
     *hit = NO_TRI;
     f3 pos = start;
@@ -108,13 +144,18 @@ __device__ __forceinline__ float propagate(const dgeom& g, f3 start, f3 dir, flo
     uint32_t cells = 0;
     do {
         float ax = g.cx, ay = g.cy, az = g.cz, aw = 1;
-        uint32_t node = 0;
+        uint32_t node = 0, level = 0;
         uint64_t child = 0;
         bool leaf = false;
+        if (pc) pc->begin();
         // descend to the node named by the location code
         for (int i = 60 - __clzll((long long)location); i >= 0; i -= 3) {
             const unsigned oct = (unsigned)(location >> i) & 7u;
-            child = g.nodes[(size_t)node * 8 + oct];
+            ++level;
+            if (!(pc && pc->hit(level, oct, child))) {
+                child = g.nodes[(size_t)node * 8 + oct];
+                if (pc) pc->miss(level, child);
+            }
             ax += aw * g.hx * ((float)(oct & 1u) - 0.5f);
             ay += aw * g.hy * ((float)((oct & 2u) >> 1) - 0.5f);
             az += aw * g.hz * ((float)((oct & 4u) >> 2) - 0.5f);
@@ -131,7 +172,11 @@ __device__ __forceinline__ float propagate(const dgeom& g, f3 start, f3 dir, flo
             oct ^= (pos.y > ay) ? 2u : 0u;
             oct ^= (pos.z > az) ? 4u : 0u;
             location = (location << 3) | oct;
-            child = g.nodes[(size_t)node * 8 + oct];
+            ++level;
+            if (!(pc && pc->hit(level, oct, child))) {
+                child = g.nodes[(size_t)node * 8 + oct];
+                if (pc) pc->miss(level, child);
+            }
             ax += aw * g.hx * ((float)(oct & 1u) - 0.5f);
             ay += aw * g.hy * ((float)((oct & 2u) >> 1) - 0.5f);
             az += aw * g.hz * ((float)((oct & 4u) >> 2) - 0.5f);
@@ -140,6 +185,7 @@ __device__ __forceinline__ float propagate(const dgeom& g, f3 start, f3 dir, flo
             leaf = (child & LEAF_BIT) != 0;
             node = (uint32_t)child;
         }
+        if (pc) pc->done(location, level, child);
         const uint32_t first = (uint32_t)child;
         const uint32_t count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
 

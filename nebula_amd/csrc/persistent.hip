@@ -131,6 +131,10 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
     const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + wave;
     const wave_stack st{ lds + (size_t)wave * 9 * P.depth, P.overflow + (size_t)gwave * 9 * P.ovf, P.depth, P.ovf };
     const uint32_t cap = P.depth + P.ovf;
+    // per-lane octree path memo after all stacks: [wave][level][lane]
+    uint32_t* path_lds = reinterpret_cast<uint32_t*>(lds + (size_t)(blockDim.x >> 6) * 9 * P.depth) +
+                         (size_t)wave * lds_path_cache::kLevels * 64 + lane;
+    lds_path_cache pcache{ path_lds, 1ull, 0ull, 0u, false };
 
     rng_xorwow rng{ rng_seed(P.rng_seed, P.rng_first + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x) };
     dparticle p{ { 0, 0, 0 }, { 0, 0, 1 }, 0 };
@@ -183,7 +187,7 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
             } else if (++steps > STEP_CAP) {
                 alive = false; ++killed;
             } else {
-                ev = init_step<true>(P.g, P.mats, P.min_e, P.max_e, p, mat, last, rng, c);
+                ev = init_step<true>(P.g, P.mats, P.min_e, P.max_e, p, mat, last, rng, c, &pcache);
                 if (ev == EV_TERMINATED) alive = false;
             }
         }
@@ -287,7 +291,7 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         require(per_cu >= 1 && per_cu <= 32, "nbl_driver_run_persistent: blocks_per_cu must be 1..32");
         require(depth >= 64 && depth <= 1024, "nbl_driver_run_persistent: stack_depth must be 64..1024");
         require(thresh >= 1 && thresh <= 64, "nbl_driver_run_persistent: refill_threshold must be 1..64");
-        const size_t lds = (size_t)(block / 64) * 9 * depth * sizeof(float);
+        const size_t lds = (size_t)(block / 64) * (9 * depth + lds_path_cache::kLevels * 64) * sizeof(float);
         require(lds <= 160 * 1024, "nbl_driver_run_persistent: stack_depth x waves exceeds the LDS size");
 
         dbuf<nbl_pixel> dpix(np);

@@ -18,9 +18,9 @@ struct lane_counters {
 // energy thresholds w.r.t. the vacuum level, free-path sampling in a
 // physical material, normalisation, octree propagation and the move.
 // p.dir is stored normalised (gpu_driver.inl:458, 480).
-template<bool COUNT, typename R>
+template<bool COUNT, typename R, typename CACHE = no_path_cache>
 __device__ __forceinline__ uint8_t init_step(const dgeom& g, const dmat* __restrict__ mats, float min_energy,
-    float max_energy, dparticle& p, int mat, uint32_t& last_tri, R& rng, lane_counters& c)
+    float max_energy, dparticle& p, int mat, uint32_t& last_tri, R& rng, lane_counters& c, CACHE* pc = nullptr)
 {
     if (COUNT) ++c.events;
     if (!in_domain(g, p.pos)) return EV_TERMINATED;
@@ -34,9 +34,9 @@ __device__ __forceinline__ uint8_t init_step(const dgeom& g, const dmat* __restr
     }
     p.dir = normalised(p.dir);
     uint32_t hit;
-    prop_counters pc{ 0, 0 };
-    const float d = propagate<COUNT>(g, p.pos, p.dir, dist, last_tri, mat, &hit, &pc);
-    if (COUNT) { c.node_steps += pc.node_steps; c.tri_tests += pc.tri_tests; }
+    prop_counters cnt{ 0, 0 };
+    const float d = propagate<COUNT, CACHE>(g, p.pos, p.dir, dist, last_tri, mat, &hit, &cnt, pc);
+    if (COUNT) { c.node_steps += cnt.node_steps; c.tri_tests += cnt.tri_tests; }
     if (hit == NO_TRI) {
         p.pos = p.pos + p.dir * dist;
         return type;                          // EV_NONE, EV_INELASTIC or EV_ELASTIC
