@@ -27,6 +27,18 @@ BYTES = dict(event=122, lookup_1d=8, lookup_2d=16, rounddown=4, node_step=4, tri
              boundary=52, secondary=65)
 
 
+WORKLOAD_TEXT = {
+    "rough": "rough surface: 512x512 Gaussian height field (sigma 2 nm, corr 10 nm, seed 1), 526334 triangles, "
+             "synthetic Si, 300 eV uniform over 1 um^2, 1e7 primaries",
+    "stripes8": "rough surface in 8 material stripes, 533502 triangles, 500 eV uniform over 1 um^2, 1e7 primaries",
+    "rough500": "rough surface, one stripe material, 500 eV uniform over 1 um^2, 1e7 primaries",
+    "lines": "Si + 5 PMMA lines (32 nm wide, 64 nm pitch, 50 nm tall), 10004 triangles, 500 eV Gaussian sigma 1 nm, "
+             "256x64 pixels x 1000 primaries",
+    "scan3": "lines geometry with 3 materials, 4096 pixels x Poisson(200) primaries, 200-2000 eV",
+    "cube": "flat Si slab (1 um cube), 1 keV pencil beam, 1e5 primaries",
+}
+
+
 def sample_top(tris):
     """Highest vertex of the sample (the detector plane excluded)."""
     s = tris[tris["material_in"] != -126]
@@ -41,19 +53,16 @@ def algorithmic_bytes(t):
             BYTES["boundary"] * t["boundary_events"] + BYTES["secondary"] * t["secondaries"])
 
 
-def cpu_baseline(tris, spec, energy, n_sample, time_limit, threads):
-    """The oracle's cpu_mt restatement on a bounded prefix of the workload."""
+def cpu_baseline(tris, specs, beam, n_sample, time_limit, threads):
+    """The oracle's cpu_mt restatement on a bounded sample of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     import runs
     import nebula_amd as na
+    from nebula_amd import workloads
     og = ol.OrcGeometry(tris)
-    om = ol.OrcMaterials([na.prepared(spec).desc])
-    rng = np.random.default_rng(12345)
-    prim = runs.pencil_primaries(n_sample, energy)
-    prim["pos"][:, 0] = rng.uniform(-500, 500, n_sample)
-    prim["pos"][:, 1] = rng.uniform(-500, 500, n_sample)
-    prim["pos"][:, 2] = sample_top(tris) + 5.0
+    om = ol.OrcMaterials([na.prepared(s).desc for s in specs])
+    prim, _ = workloads.sample_primaries(beam, n_sample, seed=12345)
     st, _ = runs.oracle_run(og, om, prim, threads=threads, time_limit=time_limit)
     return st
 
@@ -63,7 +72,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--primaries", type=int, default=10_000_000)
+    ap.add_argument("--workload", default="rough",
+                    help="rough (default: the headline config), stripes8, rough500, lines, scan3, cube")
+    ap.add_argument("--primaries", type=int, default=0, help="primaries per step (0 = the workload's own)")
     ap.add_argument("--cpu-sample", type=int, default=600_000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -80,14 +91,15 @@ def main():
         sys.exit(2)
 
     import nebula_amd as na
-    from nebula_amd import scenes, synth
-    from nebula_amd.driver import Driver, uniform_beam
+    from nebula_amd import workloads
+    from nebula_amd.driver import Driver
 
     t0 = time.time()
-    tris = scenes.rough_surface()
-    z0 = sample_top(tris) + 5.0      # above the surface, below the detector (top + 15 nm)
+    tris, specs, base_beam = workloads.BY_NAME[args.workload]()
+    base_beam = workloads.scaled(base_beam, args.primaries)
+    n_step = int(base_beam["pixels"]["n_primaries"].sum())
     g = na.Geometry(tris)
-    m = na.Materials([synth.SI])
+    m = na.Materials(specs)
     info = g.info()
     name, cus = na.device_info()
     print(f"setup: {len(tris)} triangles, {info['n_nodes']} nodes, {info['n_leaves']} leaves, "
@@ -105,7 +117,7 @@ def main():
     opts = dict(block_size=args.block_size, blocks_per_cu=args.blocks_per_cu,
                 stack_depth=args.stack_depth, refill_threshold=args.refill_threshold)
     drv = Driver(g, m, capacity=1024, seed=2024)
-    beam = lambda s: uniform_beam(300.0, args.primaries, (500.0, 500.0), z0, seed=s)
+    beam = lambda s: dict(base_beam, seed=s)
     for w in range(args.warmup):
         drv.run_persistent(beam(1000 + w), **opts)
         print(f"warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
@@ -122,7 +134,7 @@ def main():
         print(f"step {s + 1}/{args.steps}: {t['kernel_ms']:.1f} ms, SE {t['detected_se'] / t['primaries']:.4f} "
               f"BSE {t['detected_bse'] / t['primaries']:.4f}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t_start
-    assert totals["primaries"] == args.primaries * args.steps, totals
+    assert totals["primaries"] == n_step * args.steps and totals["stopped"] == 0, totals
     ms_per_step = wall * 1000.0 / args.steps
     value = totals["primaries"] / wall
     avg_kernel_ms = float(np.mean(kernel_ms))
@@ -132,11 +144,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        st = cpu_baseline(tris, synth.SI, 300.0, args.cpu_sample, args.cpu_seconds, threads)
+        st = cpu_baseline(tris, specs, base_beam, args.cpu_sample, args.cpu_seconds, threads)
         cpu = {"value": st["primaries"] / st["seconds"], "unit": "primaries/s", "cores": int(st["threads"]),
                "kind": "port",
-               "sample": f"oracle cpu_mt restatement, {st['primaries']} primaries of the same workload "
-                         f"(first {args.cpu_sample} of a uniform 300 eV beam, stopped at {args.cpu_seconds:.0f} s), "
+               "sample": f"oracle cpu_mt restatement, {st['primaries']} primaries drawn from the same pixel "
+                         f"batch (sample of {args.cpu_sample}, stopped at {args.cpu_seconds:.0f} s), "
                          f"{st['seconds']:.1f} s, {st['events'] / st['seconds']:.3g} events/s",
                "events_per_s": st["events"] / st["seconds"],
                "se_yield": st["detected_se"] / max(st["primaries"], 1),
@@ -156,10 +168,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "rough surface: 512x512 Gaussian height field (sigma 2 nm, corr 10 nm, seed 1), "
-                               "526334 triangles, synthetic Si, 300 eV uniform over 1 um^2",
-                   "primaries_per_step": args.primaries, "triangles": int(len(tris)),
-                   "sampler": "table", "kernel": "persistent"},
+        "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload),
+                   "name": args.workload, "primaries_per_step": n_step, "triangles": int(len(tris)),
+                   "materials": len(specs), "pixels": int(len(base_beam["pixels"])),
+                   "sampler": "table", "kernel": "persistent",
+                   "launch": {k: v for k, v in opts.items()}},
         "roofline": {"bound": "hbm", "achieved": ach_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": ach_gbps / HBM_PEAK_GBPS, "traffic": None,
                      "kernel": "k_persistent", "avg_kernel_ms": avg_kernel_ms, "copy_kernel_GBps": copy_gbps,

@@ -33,17 +33,28 @@ constexpr int N_COUNTERS = 14;
 // counter slots
 enum { C_PRIM = 0, C_SE, C_BSE, C_EVENTS, C_L1D, C_L2D, C_RD, C_NODES, C_TRIS, C_BOUND, C_SEC, C_PEND, C_KILLED, C_DET };
 
+constexpr uint32_t N_QUEUES = 8;       // one primary queue per XCD (blocks go round-robin over XCDs)
+constexpr uint32_t QSTRIDE = 16;       // queue counters 128 B apart
+
+// A beam spot as the kernel sees it: a caller's pixel or one tile of it.
+struct dpixel {
+    float x, y, z, energy, sigma, half_x, half_y;
+    uint32_t profile;
+    uint32_t tally;                     // index of the caller's pixel
+};
+
 struct pers_params {
     dgeom g;
     const dmat* mats;
     float min_e, max_e;
-    const nbl_pixel* pixels;
+    const dpixel* pixels;
     const uint64_t* offsets;            // n_pixels + 1 prefix sums of n_primaries
     uint32_t n_pixels;
     uint64_t total;
+    uint64_t qbeg[N_QUEUES + 1];        // primary index range of each queue
     f3 dir;
     uint64_t prim_seed, rng_seed, rng_first;
-    unsigned long long* next;
+    unsigned long long* next;           // N_QUEUES counters, QSTRIDE apart
     uint32_t* pixel_se;
     uint32_t* pixel_bse;
     unsigned long long* hist;
@@ -104,7 +115,7 @@ __device__ void make_primary(const pers_params& P, uint64_t i, dparticle& p, uin
         const uint32_t mid = (lo + hi) >> 1;
         if (P.offsets[mid] <= i) lo = mid; else hi = mid;
     }
-    const nbl_pixel px = P.pixels[lo];
+    const dpixel px = P.pixels[lo];
     const uint64_t h = mix64(P.prim_seed + 0x9E3779B97F4A7C15ull * (i + 1));
     const float u1 = (float)((uint32_t)h >> 8) * 5.9604644775390625e-8f + 2.98023223876953125e-8f;  // (0,1)
     const float u2 = (float)((uint32_t)(h >> 32) >> 8) * 5.9604644775390625e-8f;
@@ -120,7 +131,7 @@ __device__ void make_primary(const pers_params& P, uint64_t i, dparticle& p, uin
     p.pos = { x, y, px.z };
     p.dir = P.dir;
     p.e = px.energy;
-    tag = lo;
+    tag = px.tally;
 }
 
 __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
@@ -142,7 +153,8 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
     uint32_t last = NO_TRI, tag = 0, steps = 0;
     bool alive = false, pend = false;
     uint32_t sp = 0;                                // wave-uniform stack size
-    bool queue_empty = false;                       // wave-uniform
+    const uint32_t home = blockIdx.x % N_QUEUES;    // this block's XCD
+    uint32_t qi = 0;                                // queues found empty so far (wave-uniform)
     lane_counters c;
     uint32_t n_se = 0, n_bse = 0, killed = 0;
     uint64_t prims = 0;
@@ -161,20 +173,25 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
             sp -= take;
             wave_sync();
             const uint32_t need = n_idle - take;
-            if (need && !queue_empty) {
+            // fresh primaries: the home queue first, then the others in turn
+            while (need && qi < N_QUEUES) {
+                const uint32_t q = (home + qi) % N_QUEUES;
+                const uint64_t qb = P.qbeg[q], qe = P.qbeg[q + 1];
                 uint64_t base = 0;
-                if (lane == 0) base = atomicAdd(P.next, (unsigned long long)need);
-                base = bcast64(base);
-                if (base + need >= P.total) queue_empty = true;
+                if (lane == 0) base = atomicAdd(&P.next[q * QSTRIDE], (unsigned long long)need);
+                base = qb + bcast64(base);
+                if (base >= qe) { ++qi; continue; }
+                if (base + need >= qe) ++qi;
                 const uint64_t fresh = __ballot(!alive);
                 if (!alive) {
                     const uint64_t i = base + rank_below(fresh);
-                    if (i < P.total) {
+                    if (i < qe) {
                         make_primary(P, i, p, tag);
                         mat = MAT_VACUUM; last = NO_TRI; steps = 0; pend = false; alive = true;
                         ++prims;
                     }
                 }
+                break;
             }
         }
         if (__ballot(alive) == 0) break;
@@ -265,15 +282,39 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         require(batch && out, "nbl_driver_run_persistent: NULL batch or tallies");
         require(batch->n_pixels > 0 && batch->pixels, "nbl_driver_run_persistent: empty pixel batch");
         const uint32_t np = batch->n_pixels;
-        std::vector<uint64_t> offsets(np + 1, 0);
+        // Kernel pixel list: caller's pixels in order; a large uniform spot is
+        // cut into kTiles x kTiles tiles in raster order (stratified: each tile
+        // gets its share of the primaries), so that contiguous primary ranges
+        // are spatially compact.
+        constexpr uint32_t kTiles = 16;
+        std::vector<dpixel> kp;
+        std::vector<uint64_t> offsets(1, 0);
         for (uint32_t i = 0; i < np; ++i) {
             const nbl_pixel& px = batch->pixels[i];
             require(px.profile <= 1, "nbl_driver_run_persistent: pixel profile must be 0 or 1");
-            require(std::isfinite(px.x) && std::isfinite(px.y) && std::isfinite(px.z) && px.energy > 0,
-                "nbl_driver_run_persistent: pixel position/energy invalid");
-            offsets[i + 1] = offsets[i] + px.n_primaries;
+            require(std::isfinite(px.x) && std::isfinite(px.y) && std::isfinite(px.z) && px.energy > 0 &&
+                    std::isfinite(px.sigma) && std::isfinite(px.half_x) && std::isfinite(px.half_y),
+                "nbl_driver_run_persistent: pixel position/energy/size invalid");
+            if (px.n_primaries == 0) continue;
+            if (px.profile == 1 && px.n_primaries >= 64u * kTiles * kTiles) {
+                const uint32_t per = px.n_primaries / (kTiles * kTiles), extra = px.n_primaries % (kTiles * kTiles);
+                for (uint32_t ty = 0; ty < kTiles; ++ty)
+                    for (uint32_t tx = 0; tx < kTiles; ++tx) {
+                        const uint32_t k = ty * kTiles + tx;
+                        dpixel t{ px.x + px.half_x * ((float)(2 * tx + 1) / kTiles - 1.0f),
+                                  px.y + px.half_y * ((float)(2 * ty + 1) / kTiles - 1.0f), px.z, px.energy,
+                                  0.0f, px.half_x / kTiles, px.half_y / kTiles, 1u, i };
+                        kp.push_back(t);
+                        offsets.push_back(offsets.back() + per + (k < extra ? 1 : 0));
+                    }
+            } else {
+                kp.push_back(dpixel{ px.x, px.y, px.z, px.energy, px.sigma, px.half_x, px.half_y, px.profile, i });
+                offsets.push_back(offsets.back() + px.n_primaries);
+            }
         }
-        const uint64_t total = offsets[np];
+        const uint64_t total = offsets.back();
+        const uint32_t nkp = (uint32_t)kp.size();
+        require(total > 0, "nbl_driver_run_persistent: no primaries in the batch");
         const f3 dir{ batch->dir.x, batch->dir.y, batch->dir.z };
         const float dn = std::sqrt(dir.x * dir.x + dir.y * dir.y + dir.z * dir.z);
         require(dn > 0 && std::isfinite(dn), "nbl_driver_run_persistent: beam direction is zero");
@@ -294,14 +335,14 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         const size_t lds = (size_t)(block / 64) * (9 * depth + lds_path_cache::kLevels * 64) * sizeof(float);
         require(lds <= 160 * 1024, "nbl_driver_run_persistent: stack_depth x waves exceeds the LDS size");
 
-        dbuf<nbl_pixel> dpix(np);
-        dbuf<uint64_t> doff(np + 1);
-        dbuf<unsigned long long> dnext(1), dhist(NBL_HIST_BINS), dcnt(N_COUNTERS);
+        dbuf<dpixel> dpix(nkp);
+        dbuf<uint64_t> doff(nkp + 1);
+        dbuf<unsigned long long> dnext(N_QUEUES * QSTRIDE), dhist(NBL_HIST_BINS), dcnt(N_COUNTERS);
         dbuf<uint32_t> dse(np), dbse(np);
         hipStream_t s = d->stream;
-        dpix.upload(batch->pixels, np, s);
-        doff.upload(offsets.data(), np + 1, s);
-        NBL_HIP(hipMemsetAsync(dnext.p, 0, sizeof(unsigned long long), s));
+        dpix.upload(kp.data(), nkp, s);
+        doff.upload(offsets.data(), nkp + 1, s);
+        NBL_HIP(hipMemsetAsync(dnext.p, 0, N_QUEUES * QSTRIDE * sizeof(unsigned long long), s));
         NBL_HIP(hipMemsetAsync(dhist.p, 0, NBL_HIST_BINS * sizeof(unsigned long long), s));
         NBL_HIP(hipMemsetAsync(dcnt.p, 0, N_COUNTERS * sizeof(unsigned long long), s));
         NBL_HIP(hipMemsetAsync(dse.p, 0, np * sizeof(uint32_t), s));
@@ -311,7 +352,8 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         P.g = geometry_view(d->geom);
         P.mats = materials_device(d->mats);
         P.min_e = d->min_energy; P.max_e = d->max_energy;
-        P.pixels = dpix.p; P.offsets = doff.p; P.n_pixels = np; P.total = total;
+        P.pixels = dpix.p; P.offsets = doff.p; P.n_pixels = nkp; P.total = total;
+        for (uint32_t q = 0; q <= N_QUEUES; ++q) P.qbeg[q] = total * q / N_QUEUES;   // contiguous eighths
         P.dir = { dir.x / dn, dir.y / dn, dir.z / dn };
         P.prim_seed = batch->seed;
         P.rng_seed = d->seed;

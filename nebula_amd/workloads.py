@@ -56,3 +56,41 @@ def scan3(n_pixels=4096, mean=200, seed=2):
 def stripes8(n=10_000_000):
     tris = scenes.rough_surface_stripes()
     return tris, synth.stripe_materials(8), uniform_beam(500.0, n, (500.0, 500.0), _top(tris) + 5.0, seed=0)
+
+
+def rough500(n=10_000_000):
+    """The rough surface in one material at 500 eV (single-material control for stripes8)."""
+    tris = scenes.rough_surface()
+    return tris, [synth.stripe_materials(8)[0]], uniform_beam(500.0, n, (500.0, 500.0), _top(tris) + 5.0, seed=0)
+
+
+BY_NAME = {"cube": cube, "lines": lines, "rough": rough, "scan3": scan3, "stripes8": stripes8, "rough500": rough500}
+
+
+def sample_primaries(beam, n, seed=0):
+    """n primaries drawn on the host from a pixel batch (pixels in proportion
+    to their primaries, positions by the pixel's profile), as particles with
+    the pixel index as tag; used to run the same workload on the CPU."""
+    rng = np.random.default_rng(seed)
+    px = np.asarray(beam["pixels"])
+    w = px["n_primaries"].astype(float)
+    idx = rng.choice(len(px), n, p=w / w.sum())
+    p = np.zeros(n, L.PARTICLE_DTYPE)
+    x = px["x"][idx].astype(np.float64); y = px["y"][idx].astype(np.float64)
+    gauss = px["profile"][idx] == 0
+    x = np.where(gauss, x + rng.normal(0, 1, n) * px["sigma"][idx], x + px["half_x"][idx] * rng.uniform(-1, 1, n))
+    y = np.where(gauss, y + rng.normal(0, 1, n) * px["sigma"][idx], y + px["half_y"][idx] * rng.uniform(-1, 1, n))
+    p["pos"][:, 0], p["pos"][:, 1], p["pos"][:, 2] = x, y, px["z"][idx]
+    d = np.asarray(beam.get("dir", (0.0, 0.0, -1.0)), np.float64)
+    p["dir"] = d / np.linalg.norm(d)
+    p["kin_energy"] = px["energy"][idx]
+    return p, idx.astype(np.uint32)
+
+
+def scaled(beam, primaries):
+    """The same pixel batch with its primaries scaled to about `primaries` in total."""
+    px = np.array(beam["pixels"], copy=True)
+    total = px["n_primaries"].sum()
+    if primaries and total != primaries:
+        px["n_primaries"] = np.maximum(0, np.round(px["n_primaries"] * (primaries / total))).astype(np.uint32)
+    return dict(beam, pixels=px)
