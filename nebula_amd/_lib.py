@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libnbl_hip.so")
+# NBL_HIP_LIB selects another build of the library (e.g. an instrumented one)
+LIB_PATH = os.environ.get("NBL_HIP_LIB") or os.path.join(HERE, "_build", "libnbl_hip.so")
 
 NBL_OK = 0
 NBL_ERR_INVALID = -1
