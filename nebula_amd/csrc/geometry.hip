@@ -4,6 +4,7 @@
 
 #include "nbl_host.h"
 #include "octree_build.h"
+#include "wave_propagate.h"
 
 using namespace nbl;
 
@@ -44,8 +45,9 @@ int geometry_max_material(nbl_geometry g)
 
 namespace {
 
-// One ray per thread, grid-stride, with the same per-lane LDS path memo as
-// the transport kernel.  Inputs and outputs are device arrays.
+// One ray per lane, grid-stride by wavefronts, with the wavefront walk of the
+// transport kernel (shared leaf tests, per-lane LDS path memo).  Inputs and
+// outputs are device arrays.
 __global__ __launch_bounds__(256) void k_propagate(dgeom g, const f3* __restrict__ pos,
     const f3* __restrict__ dir, const float* __restrict__ dist, const uint32_t* __restrict__ ignore_tri,
     const int32_t* __restrict__ ignore_mat, uint32_t n, float* __restrict__ out_dist,
@@ -53,10 +55,15 @@ __global__ __launch_bounds__(256) void k_propagate(dgeom g, const f3* __restrict
 {
     __shared__ uint32_t memo[4 * lds_path_cache::kLevels * 64];
     lds_path_cache pc{ memo + (threadIdx.x >> 6) * lds_path_cache::kLevels * 64 + (threadIdx.x & 63), 1ull, 0ull, 0u, false };
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    // wave-uniform loop: every lane of a wavefront enters propagate_wave together
+    for (uint32_t w0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); w0 < n; w0 += gridDim.x * blockDim.x) {
+        const uint32_t i = w0 + (threadIdx.x & 63u);
+        const bool on = i < n;
+        const uint32_t j = on ? i : w0;
         uint32_t hit;
-        out_dist[i] = propagate<false>(g, pos[i], dir[i], dist[i], ignore_tri[i], ignore_mat[i], &hit, nullptr, &pc);
-        out_tri[i] = hit;
+        const float d = propagate_wave<false>(on, g, pos[j], dir[j], dist[j], ignore_tri[j], ignore_mat[j], &hit,
+                                              nullptr, &pc);
+        if (on) { out_dist[i] = d; out_tri[i] = hit; }
     }
 }
 

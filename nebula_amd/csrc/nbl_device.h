@@ -128,6 +128,115 @@ struct lds_path_cache {
     }
 };
 
+// Descent from the root to the leaf cell of the walk (octree.inl:47-84):
+// along the location code first, then by position, extending the code.
+// Cell centres accumulate exactly as the reference's: its per-level term
+// AABB.w * halfsize * (+-0.5) is the exact value +-halfsize * 2^-(level+1)
+// (products by powers of two), kept here as a half size that halves per
+// level, and added to the centre in the same order.  Returns the leaf's
+// child word; c = leaf centre, s = leaf half size (AABB.w * halfsize).
+template<bool COUNT, typename CACHE>
+__device__ __forceinline__ uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s,
+    prop_counters* cnt, CACHE* pc)
+{
+    float ax = g.cx, ay = g.cy, az = g.cz;
+    float hx = 0.5f * g.hx, hy = 0.5f * g.hy, hz = 0.5f * g.hz;
+    uint32_t node = 0, level = 0;
+    uint64_t child = 0;
+    bool leaf = false;
+    if (pc) pc->begin();
+    // along the location code (which never passes through a leaf before its last digit)
+    for (int i = 60 - __clzll((long long)location); i >= 0; i -= 3) {
+        const unsigned oct = (unsigned)(location >> i) & 7u;
+        ++level;
+        if (!(pc && pc->hit(level, oct, child))) {
+            child = g.nodes[(size_t)node * 8 + oct];
+            if (pc) pc->miss(level, child);
+        }
+        ax += (oct & 1u) ? hx : -hx;
+        ay += (oct & 2u) ? hy : -hy;
+        az += (oct & 4u) ? hz : -hz;
+        hx *= 0.5f; hy *= 0.5f; hz *= 0.5f;
+        if (COUNT) ++cnt->node_steps;
+        leaf = (child & LEAF_BIT) != 0;
+        node = (uint32_t)child;
+        if (leaf) break;
+    }
+    // by position to the leaf containing the point
+    while (!leaf) {
+        unsigned oct = 0;
+        oct ^= (pos.x > ax) ? 1u : 0u;
+        oct ^= (pos.y > ay) ? 2u : 0u;
+        oct ^= (pos.z > az) ? 4u : 0u;
+        location = (location << 3) | oct;
+        ++level;
+        if (!(pc && pc->hit(level, oct, child))) {
+            child = g.nodes[(size_t)node * 8 + oct];
+            if (pc) pc->miss(level, child);
+        }
+        ax += (oct & 1u) ? hx : -hx;
+        ay += (oct & 2u) ? hy : -hy;
+        az += (oct & 4u) ? hz : -hz;
+        hx *= 0.5f; hy *= 0.5f; hz *= 0.5f;
+        if (COUNT) ++cnt->node_steps;
+        leaf = (child & LEAF_BIT) != 0;
+        node = (uint32_t)child;
+    }
+    if (pc) pc->done(location, level, child);
+    c = { ax, ay, az };
+    s = { hx + hx, hy + hy, hz + hz };
+    return child;
+}
+
+// Exit plane of a leaf cell: AABB_intersect (octree.inl:211-220) and the
+// choice of the nearest plane (octree.inl:97-112).  axis = -1, -2, -4.
+__device__ __forceinline__ float leaf_exit(f3 c, f3 s, f3 pos, f3 dir, int& axis)
+{
+    const float tx = (c.x + copysignf(s.x + EPSILON, dir.x) - pos.x) / dir.x;
+    const float ty = (c.y + copysignf(s.y + EPSILON, dir.y) - pos.y) / dir.y;
+    const float tz = (c.z + copysignf(s.z + EPSILON, dir.z) - pos.z) / dir.z;
+    if ((tx < ty) && (tx < tz)) { axis = -1; return tx; }
+    if ((ty < tx) && (ty < tz)) { axis = -2; return ty; }
+    axis = -4;
+    return tz;
+}
+
+// One leaf candidate (octree.inl:117-147): skipped (-1) when it is the
+// ignored triangle or its material on the ray's side is the ignored one,
+// else the intersect_ray distance.
+template<bool COUNT>
+__device__ __forceinline__ float leaf_candidate(const tri_rec* __restrict__ recs, uint32_t k, f3 pos, f3 dir,
+    uint32_t ignore_tri, int ignore_mat, uint32_t& id, prop_counters* cnt)
+{
+    const float4* rp = reinterpret_cast<const float4*>(recs + k);
+    const float4 a = rp[0], b = rp[1], c = rp[2];
+    id = __float_as_uint(c.z);
+    if (id == ignore_tri) return -1;
+    const f3 r0{ a.x, a.y, a.z }, e1{ a.w, b.x, b.y }, e2{ b.z, b.w, c.x };
+    const uint32_t mats = __float_as_uint(c.y);
+    const int mat_in = (int)(int16_t)(mats & 0xFFFFu);
+    const int mat_out = (int)(int16_t)(mats >> 16);
+    const int mo = dot(cross(e1, e2), dir) < 0 ? mat_in : mat_out;
+    if (mo == ignore_mat) return -1;
+    if (COUNT) ++cnt->tri_tests;
+    return intersect_ray(r0, e1, e2, pos, dir);
+}
+
+// Step to the neighbouring cell across the exit plane (octree.inl:165-185):
+// flip the axis bit at the deepest level where that stays inside the parent.
+__device__ __forceinline__ void next_cell(uint64_t& location, int axis, f3 dir)
+{
+    const unsigned mask = (unsigned)(-axis);
+    unsigned value;
+    if (mask == 1) value = (dir.x >= 0) ? 0u : 1u;
+    else if (mask == 2) value = (dir.y >= 0) ? 0u : 2u;
+    else value = (dir.z >= 0) ? 0u : 4u;
+    while (location > 1) {
+        if ((location & mask) == value) { location ^= mask; break; }
+        location >>= 3;
+    }
+}
+
 // octree::propagate, geometry/octree.inl:32-192.  Same cell walk (location
 // codes, re-descent from the root, neighbour by bit flip), same AABB and
 // intersection arithmetic and the same "t <= best + EPSILON" replacement
@@ -143,94 +252,24 @@ __device__ __forceinline__ float propagate(const dgeom& g, f3 start, f3 dir, flo
     uint64_t location = 1;
     uint32_t cells = 0;
     do {
-        float ax = g.cx, ay = g.cy, az = g.cz, aw = 1;
-        uint32_t node = 0, level = 0;
-        uint64_t child = 0;
-        bool leaf = false;
-        if (pc) pc->begin();
-        // descend to the node named by the location code
-        for (int i = 60 - __clzll((long long)location); i >= 0; i -= 3) {
-            const unsigned oct = (unsigned)(location >> i) & 7u;
-            ++level;
-            if (!(pc && pc->hit(level, oct, child))) {
-                child = g.nodes[(size_t)node * 8 + oct];
-                if (pc) pc->miss(level, child);
-            }
-            ax += aw * g.hx * ((float)(oct & 1u) - 0.5f);
-            ay += aw * g.hy * ((float)((oct & 2u) >> 1) - 0.5f);
-            az += aw * g.hz * ((float)((oct & 4u) >> 2) - 0.5f);
-            aw *= 0.5f;
-            if (COUNT) ++cnt->node_steps;
-            leaf = (child & LEAF_BIT) != 0;
-            node = (uint32_t)child;
-            if (leaf) break;
-        }
-        // descend by position to the leaf containing the point
-        while (!leaf) {
-            unsigned oct = 0;
-            oct ^= (pos.x > ax) ? 1u : 0u;
-            oct ^= (pos.y > ay) ? 2u : 0u;
-            oct ^= (pos.z > az) ? 4u : 0u;
-            location = (location << 3) | oct;
-            ++level;
-            if (!(pc && pc->hit(level, oct, child))) {
-                child = g.nodes[(size_t)node * 8 + oct];
-                if (pc) pc->miss(level, child);
-            }
-            ax += aw * g.hx * ((float)(oct & 1u) - 0.5f);
-            ay += aw * g.hy * ((float)((oct & 2u) >> 1) - 0.5f);
-            az += aw * g.hz * ((float)((oct & 4u) >> 2) - 0.5f);
-            aw *= 0.5f;
-            if (COUNT) ++cnt->node_steps;
-            leaf = (child & LEAF_BIT) != 0;
-            node = (uint32_t)child;
-        }
-        if (pc) pc->done(location, level, child);
+        f3 c, s;
+        const uint64_t child = descend<COUNT, CACHE>(g, pos, location, c, s, cnt, pc);
         const uint32_t first = (uint32_t)child;
         const uint32_t count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
-
-        // exit plane of the leaf cell (AABB_intersect, octree.inl:211-220)
-        float best;
         int axis;          // -1, -2, -4 for x, y, z; 0 once a triangle is kept
+        float best = leaf_exit(c, s, pos, dir, axis);
         uint32_t best_id = NO_TRI;
-        {
-            const float sx = aw * g.hx, sy = aw * g.hy, sz = aw * g.hz;
-            const float tx = (ax + copysignf(sx + EPSILON, dir.x) - pos.x) / dir.x;
-            const float ty = (ay + copysignf(sy + EPSILON, dir.y) - pos.y) / dir.y;
-            const float tz = (az + copysignf(sz + EPSILON, dir.z) - pos.z) / dir.z;
-            if ((tx < ty) && (tx < tz)) { best = tx; axis = -1; }
-            else if ((ty < tx) && (ty < tz)) { best = ty; axis = -2; }
-            else { best = tz; axis = -4; }
-        }
         if (COUNT) cnt->node_steps += count + 1;
         for (uint32_t k = first; k < first + count; ++k) {
-            const float4* rp = reinterpret_cast<const float4*>(g.recs + k);
-            const float4 a = rp[0], b = rp[1], c = rp[2];
-            const uint32_t id = __float_as_uint(c.z);
-            if (id == ignore_tri) continue;
-            const f3 r0{ a.x, a.y, a.z }, e1{ a.w, b.x, b.y }, e2{ b.z, b.w, c.x };
-            const uint32_t mats = __float_as_uint(c.y);
-            const int mat_in = (int)(int16_t)(mats & 0xFFFFu);
-            const int mat_out = (int)(int16_t)(mats >> 16);
-            const int mo = dot(cross(e1, e2), dir) < 0 ? mat_in : mat_out;
-            if (mo == ignore_mat) continue;
-            if (COUNT) ++cnt->tri_tests;
-            const float t = intersect_ray(r0, e1, e2, pos, dir);
+            uint32_t id;
+            const float t = leaf_candidate<COUNT>(g.recs, k, pos, dir, ignore_tri, ignore_mat, id, cnt);
             if ((t > 0) && (t <= best + EPSILON)) { best = t; axis = 0; best_id = id; }
         }
         if (best >= to_go) return distance;
         if (axis == 0) { *hit = best_id; return distance - to_go + best; }
         to_go -= best;
         pos = pos + dir * best;
-        const unsigned mask = (unsigned)(-axis);
-        unsigned value;
-        if (mask == 1) value = (dir.x >= 0) ? 0u : 1u;
-        else if (mask == 2) value = (dir.y >= 0) ? 0u : 2u;
-        else value = (dir.z >= 0) ? 0u : 4u;
-        while (location > 1) {
-            if ((location & mask) == value) { location ^= mask; break; }
-            location >>= 3;
-        }
+        next_cell(location, axis, dir);
         if (++cells >= MAX_CELLS) return distance;
     } while (location > 1);
     return distance;

@@ -23,6 +23,7 @@
 #include <memory>
 
 #include "driver.h"
+#include "wave_propagate.h"
 
 using namespace nbl;
 
@@ -155,9 +156,11 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
     uint32_t sp = 0;                                // wave-uniform stack size
     const uint32_t home = blockIdx.x % N_QUEUES;    // this block's XCD
     uint32_t qi = 0;                                // queues found empty so far (wave-uniform)
-    lane_counters c;
-    uint32_t n_se = 0, n_bse = 0, killed = 0;
-    uint64_t prims = 0;
+    // Work counters: per-lane walk counts, and wave-uniform counts of lanes
+    // (population counts of ballots, held in scalar registers).
+    prop_counters walked{ 0, 0 };
+    uint32_t n_prims = 0, n_events = 0, n_l1d = 0, n_l2d = 0, n_rd = 0, n_bound = 0, n_sec = 0, n_pend = 0,
+             n_se = 0, n_bse = 0, n_killed = 0;
 
     for (;;) {
         // ---- refill idle lanes: stack first, then fresh primaries ----------
@@ -188,9 +191,9 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
                     if (i < qe) {
                         make_primary(P, i, p, tag);
                         mat = MAT_VACUUM; last = NO_TRI; steps = 0; pend = false; alive = true;
-                        ++prims;
                     }
                 }
+                n_prims += (uint32_t)min<uint64_t>(qe - base, (uint64_t)__popcll(fresh));
                 break;
             }
         }
@@ -198,15 +201,29 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
 
         // ---- init step: sample, propagate, move ----------------------------
         uint8_t ev = EV_NONE;
+        bool walk = false, capped = false, init = false;
+        float dist = 0;
+        uint8_t type = 0;
         if (alive) {
             if (pend) {
                 ev = EV_INELASTIC;
             } else if (++steps > STEP_CAP) {
-                alive = false; ++killed;
+                alive = false; capped = true;
             } else {
-                ev = init_step<true>(P.g, P.mats, P.min_e, P.max_e, p, mat, last, rng, c, &pcache);
-                if (ev == EV_TERMINATED) alive = false;
+                init = true;
+                lane_counters unused;
+                if (init_sample<false>(P.g, P.mats, P.min_e, P.max_e, p, mat, rng, unused, dist, type)) walk = true;
+                else alive = false;
             }
+        }
+        n_killed += __popcll(__ballot(capped));
+        n_events += __popcll(__ballot(init));
+        n_l1d += 2 * __popcll(__ballot(walk && mat >= 0));
+        {
+            // all lanes enter the wavefront walk; lanes not walking help test triangles
+            uint32_t hit;
+            const float d = propagate_wave<true>(walk, P.g, p.pos, p.dir, dist, last, mat, &hit, &walked, &pcache);
+            if (walk) ev = init_move(p, dist, d, hit, type, last);
         }
         // ---- inelastic events need room for a secondary --------------------
         // A wavefront whose live lanes all wait on a full stack goes ahead
@@ -217,20 +234,26 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
         const uint32_t room = cap - sp;
         const bool stuck = room == 0 && wm == __ballot(alive);
         const bool go = want && (stuck || rank_below(wm) < room);
-        if (want && !go) { pend = true; ++c.pending; }
+        if (want && !go) pend = true;
+        n_pend += __popcll(__ballot(want && !go));
+        n_bound += __popcll(__ballot(alive && ev == EV_INTERSECT));
+        {
+            const uint32_t n_el = __popcll(__ballot(alive && ev == EV_ELASTIC)), n_in = __popcll(__ballot(go));
+            n_l2d += n_el + n_in;
+            n_rd += n_in;
+        }
 
         // ---- event ----------------------------------------------------------
-        bool made = false;
+        bool made = false, se = false, bse = false;
         dparticle q;
         if (alive) {
             if (ev == EV_INTERSECT) {
-                ++c.boundary;
                 const tri_rec t = load_tri(P.g, last);
                 const int res = intersect_execute(P.mats, t, p, mat, rng);
                 if (res == OUT_DETECTED) {
                     alive = false;
-                    if (p.e < 50) { ++n_se; atomicAdd(&P.pixel_se[tag], 1u); }
-                    else { ++n_bse; atomicAdd(&P.pixel_bse[tag], 1u); }
+                    if (p.e < 50) { se = true; atomicAdd(&P.pixel_se[tag], 1u); }
+                    else { bse = true; atomicAdd(&P.pixel_bse[tag], 1u); }
                     const int bin = (int)p.e;
                     if (bin >= 0 && bin < NBL_HIST_BINS) atomicAdd(&P.hist[bin], 1ull);
                 } else if (res == OUT_TERMINATED) {
@@ -239,33 +262,36 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
             } else if (ev == EV_ELASTIC) {
                 last = NO_TRI;
                 elastic_execute(P.mats[mat], p, rng);
-                ++c.lookups_2d;
             } else if (go) {
                 pend = false;
                 last = NO_TRI;
                 made = inelastic_execute(P.mats[mat], p, rng, &q);
-                ++c.lookups_2d; ++c.rounddowns;
             }
         }
+        n_se += __popcll(__ballot(se));
+        n_bse += __popcll(__ballot(bse));
         // ---- push secondaries ------------------------------------------------
         const uint64_t mm = __ballot(made);
+        const uint32_t n_made = __popcll(mm);
         if (made) {
-            ++c.secondaries;
             const uint32_t j = sp + rank_below(mm);
             if (j < cap) st.put(j, q, mat, tag);
-            else ++killed;
         }
-        sp = min(sp + (uint32_t)__popcll(mm), cap);
+        n_sec += n_made;
+        n_killed += sp + n_made > cap ? sp + n_made - cap : 0u;   // dropped: no room left
+        sp = min(sp + n_made, cap);
         wave_sync();
     }
 
     // ---- counters: one atomic per wavefront per counter ----------------------
-    const uint64_t vals[N_COUNTERS] = { prims, n_se, n_bse, c.events, c.lookups_1d, c.lookups_2d, c.rounddowns,
-                                        c.node_steps, c.tri_tests, c.boundary, c.secondaries, c.pending, killed,
-                                        (uint64_t)(n_se + n_bse) };
+    const uint32_t wave_vals[N_COUNTERS] = { n_prims, n_se, n_bse, n_events, n_l1d, n_l2d, n_rd, 0, 0, n_bound,
+                                             n_sec, n_pend, n_killed, n_se + n_bse };
     for (int k = 0; k < N_COUNTERS; ++k) {
-        uint64_t v = vals[k];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        uint64_t v = wave_vals[k];
+        if (k == C_NODES || k == C_TRIS) {
+            v = k == C_NODES ? walked.node_steps : walked.tri_tests;
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        }
         if (lane == 0 && v) atomicAdd(&P.counters[k], (unsigned long long)v);
     }
 }

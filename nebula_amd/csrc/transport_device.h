@@ -18,32 +18,53 @@ struct lane_counters {
 // energy thresholds w.r.t. the vacuum level, free-path sampling in a
 // physical material, normalisation, octree propagation and the move.
 // p.dir is stored normalised (gpu_driver.inl:458, 480).
-template<bool COUNT, typename R, typename CACHE = no_path_cache>
-__device__ __forceinline__ uint8_t init_step(const dgeom& g, const dmat* __restrict__ mats, float min_energy,
-    float max_energy, dparticle& p, int mat, uint32_t& last_tri, R& rng, lane_counters& c, CACHE* pc = nullptr)
+//
+// init_sample is the part before the walk: false if the particle
+// terminates, else the distance to travel and the sampled event type.
+template<bool COUNT, typename R>
+__device__ __forceinline__ bool init_sample(const dgeom& g, const dmat* __restrict__ mats, float min_energy,
+    float max_energy, dparticle& p, int mat, R& rng, lane_counters& c, float& dist, uint8_t& type)
 {
     if (COUNT) ++c.events;
-    if (!in_domain(g, p.pos)) return EV_TERMINATED;
-    uint8_t type = 0;
-    float dist = g.max_extent;
+    if (!in_domain(g, p.pos)) return false;
+    type = 0;
+    dist = g.max_extent;
     if (mat >= 0) {
         const dmat& m = mats[mat];
-        if (p.e < m.barrier + min_energy || p.e > m.barrier + max_energy) return EV_TERMINATED;
+        if (p.e < m.barrier + min_energy || p.e > m.barrier + max_energy) return false;
         dist = sample_path(m, p.e, rng, &type);
         if (COUNT) c.lookups_1d += 2;
     }
     p.dir = normalised(p.dir);
+    return true;
+}
+
+// The move after the walk: to the sampled event (EV_NONE, EV_INELASTIC or
+// EV_ELASTIC) or to the hit triangle (EV_INTERSECT).
+__device__ __forceinline__ uint8_t init_move(dparticle& p, float dist, float travelled, uint32_t hit, uint8_t type,
+    uint32_t& last_tri)
+{
+    if (hit == NO_TRI) {
+        p.pos = p.pos + p.dir * dist;
+        return type;
+    }
+    p.pos = p.pos + p.dir * travelled;
+    last_tri = hit;
+    return EV_INTERSECT;
+}
+
+template<bool COUNT, typename R, typename CACHE = no_path_cache>
+__device__ __forceinline__ uint8_t init_step(const dgeom& g, const dmat* __restrict__ mats, float min_energy,
+    float max_energy, dparticle& p, int mat, uint32_t& last_tri, R& rng, lane_counters& c, CACHE* pc = nullptr)
+{
+    float dist;
+    uint8_t type;
+    if (!init_sample<COUNT>(g, mats, min_energy, max_energy, p, mat, rng, c, dist, type)) return EV_TERMINATED;
     uint32_t hit;
     prop_counters cnt{ 0, 0 };
     const float d = propagate<COUNT, CACHE>(g, p.pos, p.dir, dist, last_tri, mat, &hit, &cnt, pc);
     if (COUNT) { c.node_steps += cnt.node_steps; c.tri_tests += cnt.tri_tests; }
-    if (hit == NO_TRI) {
-        p.pos = p.pos + p.dir * dist;
-        return type;                          // EV_NONE, EV_INELASTIC or EV_ELASTIC
-    }
-    p.pos = p.pos + p.dir * d;
-    last_tri = hit;
-    return EV_INTERSECT;
+    return init_move(p, dist, d, hit, type, last_tri);
 }
 
 // Triangle record of an input triangle id.

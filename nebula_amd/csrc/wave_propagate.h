@@ -1,0 +1,108 @@
+// wave_propagate.h -- octree::propagate for all 64 lanes of a wavefront at
+// once, with the leaf triangle tests shared across the wavefront.
+//
+// The scalar walk (nbl_device.h:propagate) loops over its leaf's triangles;
+// in a wavefront that costs the longest leaf list of any lane, which near a
+// rough surface is ~15x the mean.  Here every lane still walks its own cells
+// (descent, exit plane, neighbour step), but the candidates of all lanes'
+// leaves are pooled and tested 64 at a time, one candidate per lane.  Each
+// lane then replays the reference's sequential rule "t > 0 && t <= best +
+// EPSILON" over its own candidates' distances in leaf order, so results are
+// identical to the scalar walk's (and the reference's), bit for bit.
+#pragma once
+
+#include "transport_device.h"
+
+namespace nbl {
+
+// Exclusive prefix sum over the 64 lanes (all lanes active); *total = sum.
+__device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& total)
+{
+    const uint32_t lane = lane_id();
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    total = __shfl(x, 63);
+    return x - v;
+}
+
+// Must be called by all 64 lanes of the wavefront (uniform control flow);
+// lanes with active == false take part in the shared tests only.
+template<bool COUNT, typename CACHE = no_path_cache>
+__device__ __forceinline__ float propagate_wave(bool active, const dgeom& g, f3 start, f3 dir, float distance,
+    uint32_t ignore_tri, int ignore_mat, uint32_t* hit, prop_counters* cnt, CACHE* pc = nullptr)
+{
+    const uint32_t lane = lane_id();
+    *hit = NO_TRI;
+    f3 pos = start;
+    float to_go = distance, result = distance;
+    uint64_t location = 1;
+    uint32_t cells = 0;
+    bool run = active;
+    while (__ballot(run)) {
+        uint32_t first = 0, count = 0;
+        int axis = 0;
+        float best = 0;
+        if (run) {
+            f3 c, s;
+            const uint64_t child = descend<COUNT, CACHE>(g, pos, location, c, s, cnt, pc);
+            first = (uint32_t)child;
+            count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
+            best = leaf_exit(c, s, pos, dir, axis);
+            if (COUNT) cnt->node_steps += count + 1;
+        }
+        uint32_t total;
+        const uint32_t off = wave_exclusive_sum(count, total);
+        uint32_t best_id = NO_TRI;
+        // candidates [off, off + count) of every lane, 64 per round
+        for (uint32_t base = 0; base < total; base += 64) {
+            const uint32_t item = base + lane;
+            // owner: the last lane whose offset is <= item (offsets ascend;
+            // the lane found has count > 0 whenever item < total)
+            uint32_t owner = 0;
+#pragma unroll
+            for (uint32_t step = 32; step > 0; step >>= 1)
+                if ((uint32_t)__shfl(off, (int)(owner + step)) <= item) owner += step;
+            const uint32_t k = __shfl(first, (int)owner) + (item - __shfl(off, (int)owner));
+            const f3 op{ __shfl(pos.x, (int)owner), __shfl(pos.y, (int)owner), __shfl(pos.z, (int)owner) };
+            const f3 od{ __shfl(dir.x, (int)owner), __shfl(dir.y, (int)owner), __shfl(dir.z, (int)owner) };
+            const uint32_t oit = __shfl(ignore_tri, (int)owner);
+            const int oim = __shfl(ignore_mat, (int)owner);
+            float t = -1;
+            uint32_t id = NO_TRI;
+            if (item < total) t = leaf_candidate<COUNT>(g.recs, k, op, od, oit, oim, id, cnt);
+            // Only candidates with t > 0 can replace best.  They are few; take
+            // them in item order (= each owner's leaf order) and let the owner
+            // apply the reference's rule to each of its own.
+            uint64_t m = __ballot(t > 0);
+            while (m) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const float tq = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), b));
+                const uint32_t idq = (uint32_t)__builtin_amdgcn_readlane((int)id, b);
+                const uint32_t it = base + b;
+                if (it >= off && it < off + count && (tq <= best + EPSILON)) { best = tq; axis = 0; best_id = idq; }
+            }
+        }
+        if (run) {
+            if (best >= to_go) {
+                run = false;
+            } else if (axis == 0) {
+                *hit = best_id;
+                result = distance - to_go + best;
+                run = false;
+            } else {
+                to_go -= best;
+                pos = pos + dir * best;
+                next_cell(location, axis, dir);
+                if (++cells >= MAX_CELLS || location <= 1) run = false;
+            }
+        }
+    }
+    return result;
+}
+
+}  // namespace nbl
