@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_propagate(dgeom g, const f3* __restrict
     uint32_t* __restrict__ out_tri)
 {
     __shared__ uint32_t memo[4 * lds_path_cache::kLevels * 64];
-    lds_path_cache pc{ memo + (threadIdx.x >> 6) * lds_path_cache::kLevels * 64 + (threadIdx.x & 63), 1ull, 0ull, 0u, false };
+    lds_path_cache pc{ memo + (threadIdx.x >> 6) * lds_path_cache::kLevels * 64 + (threadIdx.x & 63), 1ull, 0ull, 0u };
     // wave-uniform loop: every lane of a wavefront enters propagate_wave together
     for (uint32_t w0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); w0 < n; w0 += gridDim.x * blockDim.x) {
         const uint32_t i = w0 + (threadIdx.x & 63u);

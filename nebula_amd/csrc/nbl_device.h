@@ -75,116 +75,189 @@ NBL_HD bool in_domain(const dgeom& g, f3 p)   // octree.inl:24-30
            (p.y < g.cy + g.hy) && (p.z > g.cz - g.hz) && (p.z < g.cz + g.hz);
 }
 
-// triangle::intersect_ray, core/triangle.inl:7-28
+// triangle::intersect_ray, core/triangle.inl:7-28.
+// Most candidates fail the u or v range test.  Those whose barycentrics,
+// estimated with the hardware reciprocal (relative error < 2^-21), lie
+// outside the ranges by a relative margin of 2^-12 are rejected before the
+// correctly rounded divisions: for them the rounded u = fl(u_num / det) of the
+// reference is certainly outside too (the margin exceeds the estimate's error
+// by a factor of 500, and the thresholds' spacing by far more), so the result
+// stays the reference's, bit for bit.  Every other candidate takes the
+// reference's divisions and tests unchanged.
 __device__ __forceinline__ float intersect_ray(f3 r0, f3 e1, f3 e2, f3 start, f3 dir)
 {
     const f3 pvec = cross(dir, e2);
     const float det = dot(e1, pvec);
     if (fabsf(det) < EPSILON) return -1;
     const f3 tvec = start - r0;
-    const float u = dot(tvec, pvec) / det;
-    if ((u < -EPSILON) || (u > 1 + EPSILON)) return -1;
+    const float un = dot(tvec, pvec);
     const f3 qvec = cross(tvec, e1);
-    const float v = dot(dir, qvec) / det;
+    const float vn = dot(dir, qvec);
+    {
+        const float r = __builtin_amdgcn_rcpf(det);
+        const float ua = un * r, va = vn * r;
+        const float mu = fabsf(ua) * 0x1p-12f, mv = fabsf(va) * 0x1p-12f;
+        if ((ua < -EPSILON - mu) || (ua > (1 + EPSILON) + mu) || (va < -EPSILON - mv) ||
+            (ua + va > (1 + EPSILON) + (mu + mv)))
+            return -1;
+    }
+    const float u = un / det;
+    if ((u < -EPSILON) || (u > 1 + EPSILON)) return -1;
+    const float v = vn / det;
     if ((v < -EPSILON) || (u + v > 1 + EPSILON)) return -1;
     const float t = dot(e2, qvec) / det;
     if (t < 0) return -1;
     return t;
 }
 
-// Memo of the last descended path, so that the walk's re-descent from the
-// root (octree.inl:62-69 re-reads every level per cell) reloads only the
-// levels where the new path leaves the old one.  The AABB arithmetic is
-// always redone from the root, so results are identical with or without it.
-struct no_path_cache {
-    __device__ __forceinline__ void begin() {}
-    __device__ __forceinline__ bool hit(uint32_t, unsigned, uint64_t&) const { return false; }
-    __device__ __forceinline__ void miss(uint32_t, uint64_t) {}
-    __device__ __forceinline__ void done(uint64_t, uint32_t, uint64_t) {}
-};
-struct lds_path_cache {
-    static constexpr uint32_t kLevels = 16;   // internal nodes of levels 1..15 kept in LDS
-    uint32_t* node;      // this lane's column: node[l * 64] = internal node at depth l
-    uint64_t loc;        // location code of the cached leaf (1 = nothing cached)
-    uint64_t leaf;       // its child word
-    uint32_t depth;      // its depth
-    bool matched;        // the path descended so far equals the cached one
-    __device__ __forceinline__ void begin() { matched = true; }
-    // child word of depth l reached through octant oct, if the cache has it
-    __device__ __forceinline__ bool hit(uint32_t l, unsigned oct, uint64_t& child)
-    {
-        if (!matched || l > depth || ((unsigned)(loc >> (3 * (depth - l))) & 7u) != oct) { matched = false; return false; }
-        if (l == depth) { child = leaf; return true; }
-        if (l < kLevels) { child = node[l * 64]; return true; }
-        return false;                        // level not stored: load it, path still matches
-    }
-    __device__ __forceinline__ void miss(uint32_t l, uint64_t child)
-    {
-        if (l < kLevels && !(child & LEAF_BIT)) node[l * 64] = (uint32_t)child;
-    }
-    __device__ __forceinline__ void done(uint64_t location, uint32_t d, uint64_t child)
-    {
-        loc = location; depth = d; leaf = child;
-    }
-};
+// One level of the descent: the child cell's centre along each axis moves by
+// +-h, and h halves.  The reference's term AABB.w * halfsize * (+-0.5)
+// (octree.inl:64-66, 78-80) is exactly +-h (products by powers of two), and
+// it is added to the centre in the same order, so centres are bit-identical.
+__device__ __forceinline__ void cell_step(unsigned oct, f3& a, f3& h)
+{
+    a.x += (oct & 1u) ? h.x : -h.x;
+    a.y += (oct & 2u) ? h.y : -h.y;
+    a.z += (oct & 4u) ? h.z : -h.z;
+    h.x *= 0.5f; h.y *= 0.5f; h.z *= 0.5f;
+}
+__device__ __forceinline__ unsigned position_octant(f3 pos, f3 a)
+{
+    unsigned oct = 0;
+    oct ^= (pos.x > a.x) ? 1u : 0u;
+    oct ^= (pos.y > a.y) ? 2u : 0u;
+    oct ^= (pos.z > a.z) ? 4u : 0u;
+    return oct;
+}
 
 // Descent from the root to the leaf cell of the walk (octree.inl:47-84):
-// along the location code first, then by position, extending the code.
-// Cell centres accumulate exactly as the reference's: its per-level term
-// AABB.w * halfsize * (+-0.5) is the exact value +-halfsize * 2^-(level+1)
-// (products by powers of two), kept here as a half size that halves per
-// level, and added to the centre in the same order.  Returns the leaf's
+// along the location code first (it never passes through a leaf before its
+// last digit), then by position, extending the code.  Returns the leaf's
 // child word; c = leaf centre, s = leaf half size (AABB.w * halfsize).
-template<bool COUNT, typename CACHE>
+struct no_path_cache {};
+template<bool COUNT>
 __device__ __forceinline__ uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s,
-    prop_counters* cnt, CACHE* pc)
+    prop_counters* cnt, no_path_cache*)
 {
-    float ax = g.cx, ay = g.cy, az = g.cz;
-    float hx = 0.5f * g.hx, hy = 0.5f * g.hy, hz = 0.5f * g.hz;
-    uint32_t node = 0, level = 0;
+    f3 a{ g.cx, g.cy, g.cz }, h{ 0.5f * g.hx, 0.5f * g.hy, 0.5f * g.hz };
+    uint32_t node = 0;
     uint64_t child = 0;
     bool leaf = false;
-    if (pc) pc->begin();
-    // along the location code (which never passes through a leaf before its last digit)
     for (int i = 60 - __clzll((long long)location); i >= 0; i -= 3) {
         const unsigned oct = (unsigned)(location >> i) & 7u;
-        ++level;
-        if (!(pc && pc->hit(level, oct, child))) {
-            child = g.nodes[(size_t)node * 8 + oct];
-            if (pc) pc->miss(level, child);
-        }
-        ax += (oct & 1u) ? hx : -hx;
-        ay += (oct & 2u) ? hy : -hy;
-        az += (oct & 4u) ? hz : -hz;
-        hx *= 0.5f; hy *= 0.5f; hz *= 0.5f;
+        child = g.nodes[(size_t)node * 8 + oct];
+        cell_step(oct, a, h);
         if (COUNT) ++cnt->node_steps;
         leaf = (child & LEAF_BIT) != 0;
         node = (uint32_t)child;
         if (leaf) break;
     }
-    // by position to the leaf containing the point
     while (!leaf) {
-        unsigned oct = 0;
-        oct ^= (pos.x > ax) ? 1u : 0u;
-        oct ^= (pos.y > ay) ? 2u : 0u;
-        oct ^= (pos.z > az) ? 4u : 0u;
+        const unsigned oct = position_octant(pos, a);
         location = (location << 3) | oct;
-        ++level;
-        if (!(pc && pc->hit(level, oct, child))) {
-            child = g.nodes[(size_t)node * 8 + oct];
-            if (pc) pc->miss(level, child);
-        }
-        ax += (oct & 1u) ? hx : -hx;
-        ay += (oct & 2u) ? hy : -hy;
-        az += (oct & 4u) ? hz : -hz;
-        hx *= 0.5f; hy *= 0.5f; hz *= 0.5f;
+        child = g.nodes[(size_t)node * 8 + oct];
+        cell_step(oct, a, h);
         if (COUNT) ++cnt->node_steps;
         leaf = (child & LEAF_BIT) != 0;
         node = (uint32_t)child;
     }
-    if (pc) pc->done(location, level, child);
-    c = { ax, ay, az };
-    s = { hx + hx, hy + hy, hz + hz };
+    c = a;
+    s = { h.x + h.x, h.y + h.y, h.z + h.z };
+    return child;
+}
+
+// Memo of the lane's last descended path (the walk re-descends from the
+// root for every cell, octree.inl:47-84).  Levels where the new path equals
+// the memo's cost centre arithmetic only; the node where the paths part is
+// read once from LDS, and only the levels below it are loaded from HBM.
+// Internal nodes of levels 1..kLevels-1 are kept in LDS, the leaf word in
+// registers.  Centres are always recomputed, so results do not depend on it.
+struct lds_path_cache {
+    static constexpr uint32_t kLevels = 16;
+    uint32_t* node;      // this lane's column: node[l * 64] = internal node at depth l
+    uint64_t loc;        // location code of the memo's leaf (1 = empty)
+    uint64_t leaf;       // the leaf's child word
+    uint32_t depth;      // the leaf's depth (0 = empty)
+    __device__ __forceinline__ unsigned digit(uint32_t l) const { return (unsigned)(loc >> (3 * (depth - l))) & 7u; }
+    __device__ __forceinline__ void store(uint32_t l, uint64_t child)
+    {
+        if (l < kLevels && !(child & LEAF_BIT)) node[l * 64] = (uint32_t)child;
+    }
+    // internal node at depth l < depth of the memo's path
+    __device__ __forceinline__ uint32_t node_at(const dgeom& g, uint32_t l) const
+    {
+        const uint32_t k = l < kLevels ? l : kLevels - 1;
+        uint32_t n = k ? node[k * 64] : 0u;
+        for (uint32_t j = k + 1; j <= l; ++j) n = (uint32_t)g.nodes[(size_t)n * 8 + digit(j)];
+        return n;
+    }
+};
+template<bool COUNT>
+__device__ __forceinline__ uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s,
+    prop_counters* cnt, lds_path_cache* pc)
+{
+    f3 a{ g.cx, g.cy, g.cz }, h{ 0.5f * g.hx, 0.5f * g.hy, 0.5f * g.hz };
+    const uint32_t D = (uint32_t)(63 - __clzll((long long)location)) / 3;   // digits of the code
+    const uint32_t M = pc->depth;
+    // leading digits the code shares with the memo's leaf
+    uint32_t common;
+    {
+        const uint32_t md = min(D, M);
+        const uint64_t x = (location >> (3 * (D - md))) ^ (pc->loc >> (3 * (M - md)));
+        common = x == 0 ? md : md - 1 - (uint32_t)(63 - __clzll((long long)x)) / 3;
+    }
+    const bool same = M > 0 && common == D && D == M;
+    if (!same && M > 0 && common == M) common = M - 1;   // a leaf is never inside the code's path
+    uint32_t level = 0;
+    const uint32_t shared = same ? D : common;
+    while (level < shared) {
+        ++level;
+        cell_step((unsigned)(location >> (3 * (D - level))) & 7u, a, h);
+    }
+    if (COUNT) cnt->node_steps += shared;
+    uint64_t child = 0;
+    uint32_t node = 0;
+    bool leaf = false, matched = false;
+    if (same) {
+        child = pc->leaf;
+        leaf = true;
+    } else if (common == D) {
+        matched = true;                        // the code's node lies on the memo's path
+    } else {
+        node = pc->node_at(g, common);
+        while (level < D) {
+            ++level;
+            const unsigned oct = (unsigned)(location >> (3 * (D - level))) & 7u;
+            child = g.nodes[(size_t)node * 8 + oct];
+            pc->store(level, child);
+            cell_step(oct, a, h);
+            if (COUNT) ++cnt->node_steps;
+            node = (uint32_t)child;
+            leaf = (child & LEAF_BIT) != 0;
+            if (leaf) break;
+        }
+    }
+    while (!leaf) {
+        const unsigned oct = position_octant(pos, a);
+        location = (location << 3) | oct;
+        ++level;
+        if (COUNT) ++cnt->node_steps;
+        if (matched && level <= M && pc->digit(level) == oct) {
+            if (level == M) { child = pc->leaf; leaf = true; }
+        } else {
+            if (matched) { node = pc->node_at(g, level - 1); matched = false; }
+            child = g.nodes[(size_t)node * 8 + oct];
+            pc->store(level, child);
+            node = (uint32_t)child;
+            leaf = (child & LEAF_BIT) != 0;
+        }
+        cell_step(oct, a, h);
+    }
+    pc->loc = location;
+    pc->depth = level;
+    pc->leaf = child;
+    c = a;
+    s = { h.x + h.x, h.y + h.y, h.z + h.z };
     return child;
 }
 
@@ -253,7 +326,7 @@ __device__ __forceinline__ float propagate(const dgeom& g, f3 start, f3 dir, flo
     uint32_t cells = 0;
     do {
         f3 c, s;
-        const uint64_t child = descend<COUNT, CACHE>(g, pos, location, c, s, cnt, pc);
+        const uint64_t child = descend<COUNT>(g, pos, location, c, s, cnt, pc);
         const uint32_t first = (uint32_t)child;
         const uint32_t count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
         int axis;          // -1, -2, -4 for x, y, z; 0 once a triangle is kept

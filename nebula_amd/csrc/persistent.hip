@@ -146,7 +146,7 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
     // per-lane octree path memo after all stacks: [wave][level][lane]
     uint32_t* path_lds = reinterpret_cast<uint32_t*>(lds + (size_t)(blockDim.x >> 6) * 9 * P.depth) +
                          (size_t)wave * lds_path_cache::kLevels * 64 + lane;
-    lds_path_cache pcache{ path_lds, 1ull, 0ull, 0u, false };
+    lds_path_cache pcache{ path_lds, 1ull, 0ull, 0u };
 
     rng_xorwow rng{ rng_seed(P.rng_seed, P.rng_first + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x) };
     dparticle p{ { 0, 0, 0 }, { 0, 0, 1 }, 0 };

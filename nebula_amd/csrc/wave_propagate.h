@@ -48,7 +48,7 @@ __device__ __forceinline__ float propagate_wave(bool active, const dgeom& g, f3 
         float best = 0;
         if (run) {
             f3 c, s;
-            const uint64_t child = descend<COUNT, CACHE>(g, pos, location, c, s, cnt, pc);
+            const uint64_t child = descend<COUNT>(g, pos, location, c, s, cnt, pc);
             first = (uint32_t)child;
             count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
             best = leaf_exit(c, s, pos, dir, axis);
