@@ -494,27 +494,36 @@ __device__ __forceinline__ float sample_path(const dmat& m, float kin_energy, R&
     return best;
 }
 
-// kieft_elastic::execute, physics/kieft/elastic.h:67-115 (phonon + recoil loss on)
-template<typename R>
-__device__ __forceinline__ void elastic_execute(const dmat& m, dparticle& p, R& rng)
-{
-    const float x = logf(p.e);
-    const float y = rng.unit();
-    const float cos_theta = clampr(tab2_get(m.el_icdf, x, y), -1, 1);
-    const float sin_theta = sqrtf(1 - cos_theta * cos_theta);
-    p.dir = normalised(p.dir);
-    const f3 nd = normalised(make_normal_vec(p.dir, rng_phi(rng)));
-    p.dir = p.dir * cos_theta + nd * sin_theta;
-    if (p.e < 200) p.e -= minr(50e-3f, m.phonon_loss);
-    else p.e -= m.recoil_const * (1 - cos_theta) * p.e;
-}
+// Kieft elastic and inelastic scattering, split in three parts so that a
+// wavefront with lanes of both kinds runs the shared middle part (the new
+// direction frame) once:
+//   scatter_begin  the ICDF table draw and everything before the frame;
+//                  false when no rotation follows (inelastic phonon loss);
+//   scatter_frame  p.dir normalised and the unit normal nd at a random
+//                  azimuth, make_normal_vec (vector_math.h:171-186);
+//   scatter_end    the rotation, energy loss and (inelastic) the secondary.
+// Each lane draws its uniforms in the reference's order:
+//   elastic    P(cos theta), phi
+//   inelastic  P(omega'), P(binding), U(omega), phi, secondary (cos, phi).
+struct scatter_state {
+    float cos_theta;           // elastic
+    float omega, binding;      // inelastic
+    bool elastic;
+};
 
-// kieft_inelastic::execute, physics/kieft/inelastic.h:68-206 (all options on).
-// Returns true and fills *sec when a secondary electron is created.
+// kieft_elastic::execute, physics/kieft/elastic.h:67-87;
+// kieft_inelastic::execute, physics/kieft/inelastic.h:68-150.
 template<typename R>
-__device__ __forceinline__ bool inelastic_execute(const dmat& m, dparticle& p, R& rng, dparticle* sec)
+__device__ __forceinline__ bool scatter_begin(const dmat& m, dparticle& p, R& rng, bool elastic, scatter_state& s)
 {
-    const float omega0 = expf(tab2_get(m.in_icdf, logf(p.e), rng.unit()));
+    s.elastic = elastic;
+    const float y = rng.unit();
+    const float v = tab2_get(elastic ? m.el_icdf : m.in_icdf, logf(p.e), y);
+    if (elastic) {
+        s.cos_theta = clampr(v, -1, 1);
+        return true;
+    }
+    const float omega0 = expf(v);
     float binding = tab2_rounddown(m.ion, logf(omega0), rng.unit());
     float omega;
     {
@@ -544,9 +553,33 @@ __device__ __forceinline__ bool inelastic_execute(const dmat& m, dparticle& p, R
             return false;
         }
     }
-    binding = maxr(0, binding);
+    s.omega = omega;
+    s.binding = maxr(0, binding);
+    return true;
+}
+
+template<typename R>
+__device__ __forceinline__ f3 scatter_frame(dparticle& p, R& rng)
+{
     p.dir = normalised(p.dir);
-    const f3 nd = normalised(make_normal_vec(p.dir, rng_phi(rng)));
+    return normalised(make_normal_vec(p.dir, rng_phi(rng)));
+}
+
+// elastic.h:88-115 (phonon + recoil loss on); inelastic.h:151-206.
+// Returns true and fills *sec when a secondary electron is created.
+template<typename R>
+__device__ __forceinline__ bool scatter_end(const dmat& m, dparticle& p, R& rng, const scatter_state& s, f3 nd,
+    dparticle* sec)
+{
+    if (s.elastic) {
+        const float cos_theta = s.cos_theta;
+        const float sin_theta = sqrtf(1 - cos_theta * cos_theta);
+        p.dir = p.dir * cos_theta + nd * sin_theta;
+        if (p.e < 200) p.e -= minr(50e-3f, m.phonon_loss);
+        else p.e -= m.recoil_const * (1 - cos_theta) * p.e;
+        return false;
+    }
+    const float binding = s.binding, omega = s.omega;
     const float K = p.e - m.fermi + 2 * binding;
     const float dK = binding + omega;
     const float cos_theta = sqrtf(dK / K);
@@ -562,6 +595,27 @@ __device__ __forceinline__ bool inelastic_execute(const dmat& m, dparticle& p, R
     p.e -= omega;
     p.dir = p.dir - cos_theta * sdir;
     return true;
+}
+
+// kieft_elastic::execute, physics/kieft/elastic.h:67-115
+template<typename R>
+__device__ __forceinline__ void elastic_execute(const dmat& m, dparticle& p, R& rng)
+{
+    scatter_state s;
+    scatter_begin(m, p, rng, true, s);
+    const f3 nd = scatter_frame(p, rng);
+    scatter_end(m, p, rng, s, nd, nullptr);
+}
+
+// kieft_inelastic::execute, physics/kieft/inelastic.h:68-206 (all options on).
+// Returns true and fills *sec when a secondary electron is created.
+template<typename R>
+__device__ __forceinline__ bool inelastic_execute(const dmat& m, dparticle& p, R& rng, dparticle* sec)
+{
+    scatter_state s;
+    if (!scatter_begin(m, p, rng, false, s)) return false;
+    const f3 nd = scatter_frame(p, rng);
+    return scatter_end(m, p, rng, s, nd, sec);
 }
 
 constexpr int OUT_ALIVE = 0, OUT_DETECTED = 1, OUT_TERMINATED = 2;

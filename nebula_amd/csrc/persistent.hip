@@ -259,14 +259,21 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
                 } else if (res == OUT_TERMINATED) {
                     alive = false;
                 }
-            } else if (ev == EV_ELASTIC) {
-                last = NO_TRI;
-                elastic_execute(P.mats[mat], p, rng);
-            } else if (go) {
+            }
+        }
+        {
+            // elastic and inelastic lanes share one pass through each part
+            const bool el = alive && ev == EV_ELASTIC;
+            bool rot = false;
+            scatter_state ss;
+            if (el || go) {
                 pend = false;
                 last = NO_TRI;
-                made = inelastic_execute(P.mats[mat], p, rng, &q);
+                rot = scatter_begin(P.mats[mat], p, rng, el, ss);
             }
+            f3 nd{ 0, 0, 0 };
+            if (rot) nd = scatter_frame(p, rng);
+            if (rot) made = scatter_end(P.mats[mat], p, rng, ss, nd, &q);
         }
         n_se += __popcll(__ballot(se));
         n_bse += __popcll(__ballot(bse));
