@@ -153,6 +153,11 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
     int mat = MAT_VACUUM;
     uint32_t last = NO_TRI, tag = 0, steps = 0;
     bool alive = false, pend = false;
+    // a step's walk may span several loop iterations, one cell per iteration
+    bool walking = false;
+    walk_state w = walk_begin(p.pos, 0);
+    float dist = 0;
+    uint8_t type = 0;
     uint32_t sp = 0;                                // wave-uniform stack size
     const uint32_t home = blockIdx.x % N_QUEUES;    // this block's XCD
     uint32_t qi = 0;                                // queues found empty so far (wave-uniform)
@@ -199,12 +204,13 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
         }
         if (__ballot(alive) == 0) break;
 
-        // ---- init step: sample, propagate, move ----------------------------
+        // ---- init step: sample; walk one cell; move when the walk ends ------
+        // A lane whose walk needs more cells keeps walking next iteration,
+        // while the other lanes go on with their events and next steps, so
+        // every iteration walks at most one cell per lane.
         uint8_t ev = EV_NONE;
-        bool walk = false, capped = false, init = false;
-        float dist = 0;
-        uint8_t type = 0;
-        if (alive) {
+        bool started = false, capped = false, init = false;
+        if (alive && !walking) {
             if (pend) {
                 ev = EV_INELASTIC;
             } else if (++steps > STEP_CAP) {
@@ -212,18 +218,25 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
             } else {
                 init = true;
                 lane_counters unused;
-                if (init_sample<false>(P.g, P.mats, P.min_e, P.max_e, p, mat, rng, unused, dist, type)) walk = true;
-                else alive = false;
+                if (init_sample<false>(P.g, P.mats, P.min_e, P.max_e, p, mat, rng, unused, dist, type)) {
+                    started = walking = true;
+                    w = walk_begin(p.pos, dist);
+                } else {
+                    alive = false;
+                }
             }
         }
         n_killed += __popcll(__ballot(capped));
         n_events += __popcll(__ballot(init));
-        n_l1d += 2 * __popcll(__ballot(walk && mat >= 0));
+        n_l1d += 2 * __popcll(__ballot(started && mat >= 0));
         {
-            // all lanes enter the wavefront walk; lanes not walking help test triangles
-            uint32_t hit;
-            const float d = propagate_wave<true>(walk, P.g, p.pos, p.dir, dist, last, mat, &hit, &walked, &pcache);
-            if (walk) ev = init_move(p, dist, d, hit, type, last);
+            // all lanes enter the cell step; lanes not walking help test triangles
+            uint32_t hit = NO_TRI;
+            float d = 0;
+            if (walk_cell<true>(walking, P.g, w, p.dir, dist, last, mat, &hit, &d, &walked, &pcache)) {
+                walking = false;
+                ev = init_move(p, dist, d, hit, type, last);
+            }
         }
         // ---- inelastic events need room for a secondary --------------------
         // A wavefront whose live lanes all wait on a full stack goes ahead

@@ -29,78 +29,106 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& tot
     return x - v;
 }
 
-// Must be called by all 64 lanes of the wavefront (uniform control flow);
-// lanes with active == false take part in the shared tests only.
+// A lane's walk between cells (the state octree::propagate keeps across
+// iterations of its cell loop, octree.inl:32-190).
+struct walk_state {
+    f3 pos;              // current point
+    float to_go;         // distance left
+    uint64_t location;   // code of the next cell
+    uint32_t cells;      // cells visited
+};
+
+__device__ __forceinline__ walk_state walk_begin(f3 start, float distance)
+{
+    return walk_state{ start, distance, 1ull, 0u };
+}
+
+// One cell of the walk of every lane with run == true; must be called by all
+// 64 lanes of the wavefront (uniform control flow); lanes not running take
+// part in the shared triangle tests only.  Returns true for lanes whose walk
+// ended in this cell, with *hit (a triangle's input index or NO_TRI) and
+// *travelled (propagate's return value for a walk of `distance`).
+template<bool COUNT, typename CACHE = no_path_cache>
+__device__ __forceinline__ bool walk_cell(bool run, const dgeom& g, walk_state& w, f3 dir, float distance,
+    uint32_t ignore_tri, int ignore_mat, uint32_t* hit, float* travelled, prop_counters* cnt, CACHE* pc = nullptr)
+{
+    const uint32_t lane = lane_id();
+    uint32_t first = 0, count = 0;
+    int axis = 0;
+    float best = 0;
+    if (run) {
+        f3 c, s;
+        const uint64_t child = descend<COUNT>(g, w.pos, w.location, c, s, cnt, pc);
+        first = (uint32_t)child;
+        count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
+        best = leaf_exit(c, s, w.pos, dir, axis);
+        if (COUNT) cnt->node_steps += count + 1;
+    }
+    uint32_t total;
+    const uint32_t off = wave_exclusive_sum(count, total);
+    uint32_t best_id = NO_TRI;
+    // candidates [off, off + count) of every lane, 64 per round
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t item = base + lane;
+        // owner: the last lane whose offset is <= item (offsets ascend; the
+        // lane found has count > 0 whenever item < total)
+        uint32_t owner = 0;
+#pragma unroll
+        for (uint32_t step = 32; step > 0; step >>= 1)
+            if ((uint32_t)__shfl(off, (int)(owner + step)) <= item) owner += step;
+        const uint32_t k = __shfl(first, (int)owner) + (item - __shfl(off, (int)owner));
+        const f3 op{ __shfl(w.pos.x, (int)owner), __shfl(w.pos.y, (int)owner), __shfl(w.pos.z, (int)owner) };
+        const f3 od{ __shfl(dir.x, (int)owner), __shfl(dir.y, (int)owner), __shfl(dir.z, (int)owner) };
+        const uint32_t oit = __shfl(ignore_tri, (int)owner);
+        const int oim = __shfl(ignore_mat, (int)owner);
+        float t = -1;
+        uint32_t id = NO_TRI;
+        if (item < total) t = leaf_candidate<COUNT>(g.recs, k, op, od, oit, oim, id, cnt);
+        // Only candidates with t > 0 can replace best.  They are few; take
+        // them in item order (= each owner's leaf order) and let the owner
+        // apply the reference's rule to each of its own.
+        uint64_t m = __ballot(t > 0);
+        while (m) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            const float tq = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), b));
+            const uint32_t idq = (uint32_t)__builtin_amdgcn_readlane((int)id, b);
+            const uint32_t it = base + b;
+            if (it >= off && it < off + count && (tq <= best + EPSILON)) { best = tq; axis = 0; best_id = idq; }
+        }
+    }
+    if (!run) return false;
+    if (best >= w.to_go) {
+        *hit = NO_TRI; *travelled = distance;
+        return true;
+    }
+    if (axis == 0) {
+        *hit = best_id; *travelled = distance - w.to_go + best;
+        return true;
+    }
+    w.to_go -= best;
+    w.pos = w.pos + dir * best;
+    next_cell(w.location, axis, dir);
+    if (++w.cells >= MAX_CELLS || w.location <= 1) {
+        *hit = NO_TRI; *travelled = distance;
+        return true;
+    }
+    return false;
+}
+
+// octree::propagate for all 64 lanes at once (all must call it); lanes with
+// active == false take part in the shared tests only.
 template<bool COUNT, typename CACHE = no_path_cache>
 __device__ __forceinline__ float propagate_wave(bool active, const dgeom& g, f3 start, f3 dir, float distance,
     uint32_t ignore_tri, int ignore_mat, uint32_t* hit, prop_counters* cnt, CACHE* pc = nullptr)
 {
-    const uint32_t lane = lane_id();
+    walk_state w = walk_begin(start, distance);
+    float result = distance;
     *hit = NO_TRI;
-    f3 pos = start;
-    float to_go = distance, result = distance;
-    uint64_t location = 1;
-    uint32_t cells = 0;
     bool run = active;
     while (__ballot(run)) {
-        uint32_t first = 0, count = 0;
-        int axis = 0;
-        float best = 0;
-        if (run) {
-            f3 c, s;
-            const uint64_t child = descend<COUNT>(g, pos, location, c, s, cnt, pc);
-            first = (uint32_t)child;
-            count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
-            best = leaf_exit(c, s, pos, dir, axis);
-            if (COUNT) cnt->node_steps += count + 1;
-        }
-        uint32_t total;
-        const uint32_t off = wave_exclusive_sum(count, total);
-        uint32_t best_id = NO_TRI;
-        // candidates [off, off + count) of every lane, 64 per round
-        for (uint32_t base = 0; base < total; base += 64) {
-            const uint32_t item = base + lane;
-            // owner: the last lane whose offset is <= item (offsets ascend;
-            // the lane found has count > 0 whenever item < total)
-            uint32_t owner = 0;
-#pragma unroll
-            for (uint32_t step = 32; step > 0; step >>= 1)
-                if ((uint32_t)__shfl(off, (int)(owner + step)) <= item) owner += step;
-            const uint32_t k = __shfl(first, (int)owner) + (item - __shfl(off, (int)owner));
-            const f3 op{ __shfl(pos.x, (int)owner), __shfl(pos.y, (int)owner), __shfl(pos.z, (int)owner) };
-            const f3 od{ __shfl(dir.x, (int)owner), __shfl(dir.y, (int)owner), __shfl(dir.z, (int)owner) };
-            const uint32_t oit = __shfl(ignore_tri, (int)owner);
-            const int oim = __shfl(ignore_mat, (int)owner);
-            float t = -1;
-            uint32_t id = NO_TRI;
-            if (item < total) t = leaf_candidate<COUNT>(g.recs, k, op, od, oit, oim, id, cnt);
-            // Only candidates with t > 0 can replace best.  They are few; take
-            // them in item order (= each owner's leaf order) and let the owner
-            // apply the reference's rule to each of its own.
-            uint64_t m = __ballot(t > 0);
-            while (m) {
-                const uint32_t b = (uint32_t)__builtin_ctzll(m);
-                m &= m - 1;
-                const float tq = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), b));
-                const uint32_t idq = (uint32_t)__builtin_amdgcn_readlane((int)id, b);
-                const uint32_t it = base + b;
-                if (it >= off && it < off + count && (tq <= best + EPSILON)) { best = tq; axis = 0; best_id = idq; }
-            }
-        }
-        if (run) {
-            if (best >= to_go) {
-                run = false;
-            } else if (axis == 0) {
-                *hit = best_id;
-                result = distance - to_go + best;
-                run = false;
-            } else {
-                to_go -= best;
-                pos = pos + dir * best;
-                next_cell(location, axis, dir);
-                if (++cells >= MAX_CELLS || location <= 1) run = false;
-            }
-        }
+        if (walk_cell<COUNT, CACHE>(run, g, w, dir, distance, ignore_tri, ignore_mat, hit, &result, cnt, pc))
+            run = false;
     }
     return result;
 }
