@@ -7,7 +7,14 @@ to 522,242 surface triangles (526,334 with walls, bottom and detector); 1e7
 primaries of 300 eV uniform over the 1 um^2 area.  One step = transport of
 all 1e7 primaries to the end of their cascades in one persistent launch.
 
-Output: one JSON line (metric, value, roofline, cpu_baseline, counters).
+Second part of the metric (learned vs table sampler, unless --no-learned):
+the rough surface in 8 material stripes, 1e7 primaries of 500 eV, once with
+the ICDF tables and once with the learned BF16 sampler trained on 1e8
+samples of the same tables; primaries/s of both, yields, and the sampler's
+KS distance per energy bin.
+
+Output: one JSON line (metric, value, roofline, cpu_baseline, counters,
+learned_vs_table).
 `--gpus N` with N > 1 exits: this build measures one GPU.
 """
 import argparse
@@ -39,18 +46,64 @@ WORKLOAD_TEXT = {
 }
 
 
-def sample_top(tris):
-    """Highest vertex of the sample (the detector plane excluded)."""
-    s = tris[tris["material_in"] != -126]
-    z = np.concatenate([s["r0"][:, 2], s["r0"][:, 2] + s["e1"][:, 2], s["r0"][:, 2] + s["e2"][:, 2]])
-    return float(z.max())
-
-
 def algorithmic_bytes(t):
     return (BYTES["event"] * t["events"] + BYTES["lookup_1d"] * t["lookups_1d"] +
             BYTES["lookup_2d"] * t["lookups_2d"] + BYTES["rounddown"] * t["rounddowns"] +
             BYTES["node_step"] * t["node_steps"] + BYTES["tri_test"] * t["tri_tests"] +
             BYTES["boundary"] * t["boundary_events"] + BYTES["secondary"] * t["secondaries"])
+
+
+def learned_vs_table(primaries, seed, train_samples):
+    """The 8-material stripes workload with the table path and with the
+    learned sampler (trained here on samples of the same tables): primaries/s
+    of each, yields, and the learned sampler's KS distance per energy bin."""
+    import nebula_amd as na
+    from nebula_amd import workloads
+    from nebula_amd.driver import Driver
+    tris, specs, beam = workloads.stripes8()
+    beam = workloads.scaled(beam, primaries)
+    e_beam = float(beam["pixels"]["energy"].max())
+    g = na.Geometry(tris)
+    m = na.Materials(specs)
+    t0 = time.time()
+    smp = na.Sampler(m, samples=train_samples, e_min=20.0, e_max=1.2 * e_beam, seed=seed)
+    train_s = time.time() - t0
+    rep = smp.report()
+    edges = np.geomspace(30.0, 0.95 * e_beam, 7)
+    ks = {k: [smp.ks_per_bin(kind, mat, edges, n=20000, seed=mat) for mat in range(len(specs))]
+          for k, kind in (("elastic", 0), ("inelastic", 1))}
+    out = {}
+    for name, sampler in (("table", None), ("learned", smp)):
+        with Driver(g, m, capacity=1024, seed=77) as drv:
+            drv.set_sampler(sampler)
+            drv.run_persistent(dict(beam, seed=1))                       # warm-up
+            t1 = time.perf_counter()
+            t = drv.run_persistent(dict(beam, seed=2))
+            wall = time.perf_counter() - t1
+        n = t["primaries"]
+        out[name] = {"primaries_per_s": n / wall, "kernel_ms": t["kernel_ms"], "events_per_s": t["events"] / wall,
+                     "se": t["detected_se"] / n, "bse": t["detected_bse"] / n, "stopped": t["stopped"],
+                     "counts": (t["detected_se"], t["detected_bse"], n)}
+        print(f"{name}: {n / wall:.4g} primaries/s, SE {out[name]['se']:.4f} BSE {out[name]['bse']:.4f}",
+              file=sys.stderr, flush=True)
+    z = {}
+    for i, key in enumerate(("se", "bse")):
+        (a, na_), (b, nb_) = (out["table"]["counts"][i], out["table"]["counts"][2]), \
+                             (out["learned"]["counts"][i], out["learned"]["counts"][2])
+        # Poisson counting error of each yield
+        z[key] = (b / nb_ - a / na_) / np.hypot(np.sqrt(max(a, 1)) / na_, np.sqrt(max(b, 1)) / nb_)
+    for v in out.values():
+        v.pop("counts")
+    return {"workload": WORKLOAD_TEXT["stripes8"].replace("1e7", f"{primaries:.0e}"),
+            "table": out["table"], "learned": out["learned"],
+            "speedup_learned_vs_table": out["learned"]["primaries_per_s"] / out["table"]["primaries_per_s"],
+            "yield_z": z, "yields_equal_3sigma": bool(max(abs(z["se"]), abs(z["bse"])) < 3),
+            "ks_per_bin": {"energy_edges_eV": [float(e) for e in edges],
+                           **{k: [[round(float(x), 5) for x in row] for row in v] for k, v in ks.items()},
+                           "max": {k: float(np.max(v)) for k, v in ks.items()}},
+            "training": {"samples_per_network": rep["samples"], "seconds": train_s,
+                         "mse_standardised": {"elastic": rep["loss_elastic"], "inelastic": rep["loss_inelastic"]},
+                         "energy_range_eV": [rep["e_min"], rep["e_max"]]}}
 
 
 def cpu_baseline(tris, specs, beam, n_sample, time_limit, threads):
@@ -78,6 +131,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=600_000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-table sampler comparison")
+    ap.add_argument("--learned-primaries", type=int, default=10_000_000)
+    ap.add_argument("--train-samples", type=int, default=100_000_000)
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--stack-depth", type=int, default=0)
@@ -155,6 +211,10 @@ def main():
                "bse_yield": st["detected_bse"] / max(st["primaries"], 1)}
         print(f"cpu baseline: {cpu['value']:.0f} primaries/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_vs_table(args.learned_primaries, 5, args.train_samples)
+
     out = {
         "metric": "primary electrons/s (1 MI355X) vs host CPU; learned vs table sampler at equal yield",
         "value": value,
@@ -182,6 +242,7 @@ def main():
         "events_per_s": totals["events"] / wall,
         "counters_per_step": per_launch,
         "yields": {"se": totals["detected_se"] / totals["primaries"], "bse": totals["detected_bse"] / totals["primaries"]},
+        "learned_vs_table": learned,
     }
     print(json.dumps(out), flush=True)
 

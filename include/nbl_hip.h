@@ -347,6 +347,53 @@ typedef struct {
 int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch,
     const nbl_persistent_opts* opts, nbl_tallies* out);
 
+/* ---- learned ICDF sampler (no reference counterpart: stands in for the
+ * elastic cos(theta) table of kieft_elastic, physics/kieft/elastic.h:67-87,
+ * and the inelastic omega' table of kieft_inelastic, inelastic.h:68-72) ---- */
+/* Two BF16 MLPs (elastic, inelastic) shared by all materials of a material
+ * set: features (log E, u, log u, log(1-u), 8-dim learned material
+ * embedding) -> 3 x 64 ReLU -> 1, FP32 accumulation on the matrix cores.
+ * Trained at creation on samples drawn from the material tables. */
+typedef struct nbl_sampler_s* nbl_sampler;
+#define NBL_SAMPLER_ELASTIC    0   /* value: cos(theta), as the elastic table */
+#define NBL_SAMPLER_INELASTIC  1   /* value: log(omega'), as the inelastic table */
+/* FP32 parameter block of one network, in floats:
+ *   W1 [64][16] @0, b1 [64] @1024, W2 [64][64] @1088, b2 @5184, W3 [64][64] @5248,
+ *   b3 @9344, w4 [64] @9408, b4 @9472 (+3 pad), embeddings [64 materials][8] @9476 */
+#define NBL_SAMPLER_PARAMS 9988
+typedef struct {
+    uint64_t samples;        /* training samples per network; 0 = 1e8            */
+    uint32_t batch;          /* samples per step (multiple of 256); 0 = 16384     */
+    float lr;                /* Adam learning rate (cosine decay); 0 = 2e-3       */
+    float e_min, e_max;      /* kinetic-energy range of the samples, eV; 0 = the tables' */
+    uint64_t seed;
+    uint32_t reserved[4];
+} nbl_sampler_opts;
+typedef struct {
+    double loss_elastic, loss_inelastic;   /* final mean squared error, standardised targets */
+    double train_ms;
+    uint64_t steps, samples;               /* per network */
+    double e_min, e_max;                   /* energy range trained on, eV */
+} nbl_sampler_report;
+
+/* Trains both networks for the materials of m (1..64 materials). */
+int nbl_sampler_create(nbl_materials m, const nbl_sampler_opts* opts, nbl_sampler* out);
+int nbl_sampler_destroy(nbl_sampler s);
+int nbl_sampler_get_report(nbl_sampler s, nbl_sampler_report* r);
+/* FP32 parameters (n = NBL_SAMPLER_PARAMS) and consts[4] = feature-0 shift,
+ * feature-0 scale, output shift, output scale. */
+int nbl_sampler_get_params(nbl_sampler s, int kind, float* params, uint32_t n, float* consts);
+/* The network's value for n samples (host arrays), through the matrix-core
+ * path the transport uses: cos(theta) or log(omega'). */
+int nbl_sampler_eval_batch(nbl_sampler s, int kind, const int32_t* mat, const float* energy, const float* u,
+    uint32_t n, float* out);
+/* Known-answer test of the BF16 matrix-core operand layout (one 16x16x16
+ * product); *max_abs_err is 0 on a correct device. */
+int nbl_sampler_selftest(double* max_abs_err);
+/* Use sampler s (NULL = the tables) in the driver's persistent transport.
+ * s must have been created for the driver's material set. */
+int nbl_driver_set_sampler(nbl_driver d, nbl_sampler s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@ importing the package fails if it is missing.
 """
 from . import _lib
 from ._lib import NblError
-from .api import Geometry, Materials, PreparedMaterial, octree_linearize, prepared, device_info
+from .api import Geometry, Materials, PreparedMaterial, Sampler, octree_linearize, prepared, device_info
 
-__all__ = ["NblError", "Geometry", "Materials", "PreparedMaterial", "octree_linearize", "prepared", "device_info"]
+__all__ = ["NblError", "Geometry", "Materials", "PreparedMaterial", "Sampler", "octree_linearize", "prepared",
+           "device_info"]

@@ -128,6 +128,21 @@ class PersistentOpts(C.Structure):
                 ("overflow_depth", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
+class SamplerOpts(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("batch", C.c_uint32), ("lr", C.c_float),
+                ("e_min", C.c_float), ("e_max", C.c_float), ("seed", C.c_uint64),
+                ("reserved", C.c_uint32 * 4)]
+
+
+class SamplerReport(C.Structure):
+    _fields_ = [("loss_elastic", C.c_double), ("loss_inelastic", C.c_double), ("train_ms", C.c_double),
+                ("steps", C.c_uint64), ("samples", C.c_uint64), ("e_min", C.c_double), ("e_max", C.c_double)]
+
+
+SAMPLER_ELASTIC, SAMPLER_INELASTIC = 0, 1
+SAMPLER_PARAMS = 9988
+
+
 DETECT_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_uint32, C.c_void_p)
 POOL_FILL_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p)
 
@@ -172,6 +187,13 @@ SIGNATURES = {
     "nbl_rng_seed": [_U64, _U64, _U32, _P],
     "nbl_rng_uniform_batch": [_P, _U32, _U32, _P],
     "nbl_driver_run_persistent": [_P, C.POINTER(PixelBatch), C.POINTER(PersistentOpts), C.POINTER(Tallies)],
+    "nbl_sampler_create": [_P, C.POINTER(SamplerOpts), C.POINTER(_P)],
+    "nbl_sampler_destroy": [_P],
+    "nbl_sampler_get_report": [_P, C.POINTER(SamplerReport)],
+    "nbl_sampler_get_params": [_P, _I32, _P, _U32, _P],
+    "nbl_sampler_eval_batch": [_P, _I32, _P, _P, _P, _U32, _P],
+    "nbl_sampler_selftest": [C.POINTER(C.c_double)],
+    "nbl_driver_set_sampler": [_P, _P],
 }
 
 

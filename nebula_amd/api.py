@@ -158,6 +158,79 @@ class Materials:
         return dict(particles=io, material=om, status=st, secondary=sec, has_secondary=has, scalar=sc, type=ty)
 
 
+def ks_distance(a, b) -> float:
+    """Two-sample Kolmogorov-Smirnov distance sup |F_a - F_b|."""
+    a = np.sort(np.asarray(a, np.float64)); b = np.sort(np.asarray(b, np.float64))
+    grid = np.concatenate([a, b])
+    fa = np.searchsorted(a, grid, side="right") / len(a)
+    fb = np.searchsorted(b, grid, side="right") / len(b)
+    return float(np.max(np.abs(fa - fb)))
+
+
+class Sampler:
+    """Learned ICDF sampler of a material set: two BF16 MLPs (elastic
+    cos(theta), inelastic log omega') evaluated on the matrix cores, trained
+    at construction on samples drawn from the material tables."""
+
+    KINDS = {L.SAMPLER_ELASTIC: L.TAB_ELASTIC_ICDF, L.SAMPLER_INELASTIC: L.TAB_INELASTIC_W0_ICDF}
+
+    def __init__(self, materials: Materials, samples=100_000_000, batch=16384, lr=2e-3, e_min=0.0, e_max=0.0,
+                 seed=1):
+        self.materials = materials              # the tables must outlive the sampler
+        o = L.SamplerOpts(samples=int(samples), batch=int(batch), lr=float(lr), e_min=float(e_min),
+                          e_max=float(e_max), seed=int(seed))
+        h = C.c_void_p()
+        call("nbl_sampler_create", materials.h, C.byref(o), C.byref(h))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            L.lib.nbl_sampler_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def report(self) -> dict:
+        r = L.SamplerReport()
+        call("nbl_sampler_get_report", self.h, C.byref(r))
+        return {k: getattr(r, k) for k, _ in L.SamplerReport._fields_}
+
+    def params(self, kind):
+        """(FP32 parameter block, [feature shift, feature scale, output shift, output scale])."""
+        p = np.zeros(L.SAMPLER_PARAMS, np.float32)
+        c = np.zeros(4, np.float32)
+        call("nbl_sampler_get_params", self.h, kind, ptr(p), L.SAMPLER_PARAMS, ptr(c))
+        return p, c
+
+    def eval(self, kind, material, energy, u):
+        """The network's value (cos theta or log omega') through the matrix-core path."""
+        energy = np.ascontiguousarray(energy, np.float32)
+        n = len(energy)
+        mat = np.ascontiguousarray(np.broadcast_to(np.asarray(material, np.int32), (n,)))
+        u = np.ascontiguousarray(u, np.float32)
+        out = np.zeros(n, np.float32)
+        call("nbl_sampler_eval_batch", self.h, kind, ptr(mat), ptr(energy), ptr(u), n, ptr(out))
+        return out
+
+    def table(self, kind, material, energy, u):
+        """The same quantity from the material's table (the sampler's judge)."""
+        return self.materials.table2d(material, self.KINDS[kind], np.log(np.asarray(energy, np.float32)), u)
+
+    def ks_per_bin(self, kind, material, edges, n=20000, seed=0):
+        """KS distance between learned and table samples in each energy bin
+        [edges[i], edges[i+1]) (energies log-uniform within a bin, the same
+        uniforms for both).  KS is invariant under the monotone map between
+        log omega' and omega', so either may be compared."""
+        rng = np.random.default_rng(seed)
+        out = []
+        for lo, hi in zip(edges[:-1], edges[1:]):
+            e = np.exp(rng.uniform(np.log(lo), np.log(hi), n)).astype(np.float32)
+            u = ((rng.integers(0, 1 << 24, n) + 1) * 2.0 ** -24).astype(np.float32)
+            a, b = self.table(kind, material, e, u), self.eval(kind, material, e, u)
+            out.append(ks_distance(a, b))
+        return out
+
+
 def device_info():
     name = C.create_string_buffer(128)
     cu = C.c_int()

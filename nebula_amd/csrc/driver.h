@@ -58,6 +58,7 @@ struct nbl_driver_s {
     nbl::dbuf<nbl::rng_state> rng;
     nbl::sorter sort;
     nbl::dbuf<float> overflow;          // persistent kernel: per-wavefront HBM stacks
+    nbl_sampler sampler = nullptr;      // persistent kernel: learned ICDF sampler (NULL = tables)
 
     // input buffers (gpu_driver.h:211-217)
     uint32_t in_size = 0;

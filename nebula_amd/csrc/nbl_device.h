@@ -513,12 +513,13 @@ struct scatter_state {
 
 // kieft_elastic::execute, physics/kieft/elastic.h:67-87;
 // kieft_inelastic::execute, physics/kieft/inelastic.h:68-150.
+// scatter_begin_v takes the ICDF value v of the first draw (cos theta or
+// log omega') from the caller: the table, or the learned sampler.
 template<typename R>
-__device__ __forceinline__ bool scatter_begin(const dmat& m, dparticle& p, R& rng, bool elastic, scatter_state& s)
+__device__ __forceinline__ bool scatter_begin_v(const dmat& m, dparticle& p, R& rng, bool elastic, float v,
+    scatter_state& s)
 {
     s.elastic = elastic;
-    const float y = rng.unit();
-    const float v = tab2_get(elastic ? m.el_icdf : m.in_icdf, logf(p.e), y);
     if (elastic) {
         s.cos_theta = clampr(v, -1, 1);
         return true;
@@ -556,6 +557,13 @@ __device__ __forceinline__ bool scatter_begin(const dmat& m, dparticle& p, R& rn
     s.omega = omega;
     s.binding = maxr(0, binding);
     return true;
+}
+template<typename R>
+__device__ __forceinline__ bool scatter_begin(const dmat& m, dparticle& p, R& rng, bool elastic, scatter_state& s)
+{
+    const float y = rng.unit();
+    const float v = tab2_get(elastic ? m.el_icdf : m.in_icdf, logf(p.e), y);
+    return scatter_begin_v(m, p, rng, elastic, v, s);
 }
 
 template<typename R>

@@ -23,6 +23,7 @@
 #include <memory>
 
 #include "driver.h"
+#include "sampler.h"
 #include "wave_propagate.h"
 
 using namespace nbl;
@@ -64,6 +65,7 @@ struct pers_params {
     uint32_t depth;                     // LDS stack entries per wavefront
     uint32_t ovf;                       // HBM stack entries per wavefront
     uint32_t refill_threshold;
+    dmlp mlp[2];                        // learned sampler (elastic, inelastic) when LEARNED
 };
 
 // One logical stack per wavefront: entries [0, D) in LDS, [D, D + G) in HBM.
@@ -135,7 +137,12 @@ __device__ void make_primary(const pers_params& P, uint64_t i, dparticle& p, uin
     tag = px.tally;
 }
 
-__global__ __launch_bounds__(1024) void k_persistent(pers_params P)
+// LEARNED: the elastic and inelastic ICDF draws come from the learned
+// sampler's networks (evaluated by the whole wavefront) instead of the tables.
+// The learned variant needs ~150 registers while the networks run, so it is
+// built for 12 wavefronts per CU (3 per SIMD) instead of 16.
+template<bool LEARNED>
+__global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params P)
 {
     extern __shared__ float lds[];
     const uint32_t wave = threadIdx.x >> 6;
@@ -279,7 +286,18 @@ __global__ __launch_bounds__(1024) void k_persistent(pers_params P)
             const bool el = alive && ev == EV_ELASTIC;
             bool rot = false;
             scatter_state ss;
-            if (el || go) {
+            if (LEARNED) {
+                float y = 0.5f, le = 0.0f;
+                if (el || go) { y = rng.unit(); le = logf(p.e); }
+                float v = 0;
+                if (__ballot(el)) { const float ve = mlp_eval_wave(P.mlp[MLP_KIND_ELASTIC], mat, le, y); if (el) v = ve; }
+                if (__ballot(go)) { const float vi = mlp_eval_wave(P.mlp[MLP_KIND_INELASTIC], mat, le, y); if (go) v = vi; }
+                if (el || go) {
+                    pend = false;
+                    last = NO_TRI;
+                    rot = scatter_begin_v(P.mats[mat], p, rng, el, v, ss);
+                }
+            } else if (el || go) {
                 pend = false;
                 last = NO_TRI;
                 rot = scatter_begin(P.mats[mat], p, rng, el, ss);
@@ -369,12 +387,14 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         NBL_HIP(hipGetDevice(&dev));
         NBL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         uint32_t block = opts && opts->block_size ? opts->block_size : 256;
-        uint32_t per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : 4;
+        // the learned variant holds 3 wavefronts per SIMD (12 per CU), the table variant 4
+        uint32_t per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : (d->sampler ? 3u : 4u);
         uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 128;
         uint32_t thresh = opts && opts->refill_threshold ? opts->refill_threshold : 1;
         uint32_t ovf = opts && opts->overflow_depth ? opts->overflow_depth : 4096;
         require(ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
         require(block % 64 == 0 && block >= 64 && block <= 1024, "nbl_driver_run_persistent: block_size must be 64..1024, multiple of 64");
+        require(!d->sampler || block <= 768, "nbl_driver_run_persistent: block_size must be <= 768 with a learned sampler");
         require(per_cu >= 1 && per_cu <= 32, "nbl_driver_run_persistent: blocks_per_cu must be 1..32");
         require(depth >= 64 && depth <= 1024, "nbl_driver_run_persistent: stack_depth must be 64..1024");
         require(thresh >= 1 && thresh <= 64, "nbl_driver_run_persistent: refill_threshold must be 1..64");
@@ -415,7 +435,13 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         hipEvent_t e0, e1;
         NBL_HIP(hipEventCreate(&e0)); NBL_HIP(hipEventCreate(&e1));
         NBL_HIP(hipEventRecord(e0, s));
-        hipLaunchKernelGGL(k_persistent, dim3((unsigned)(cus * per_cu)), dim3(block), lds, s, P);
+        if (d->sampler) {
+            const dmlp* v = sampler_views(d->sampler);
+            P.mlp[0] = v[0]; P.mlp[1] = v[1];
+            hipLaunchKernelGGL(k_persistent<true>, dim3((unsigned)(cus * per_cu)), dim3(block), lds, s, P);
+        } else {
+            hipLaunchKernelGGL(k_persistent<false>, dim3((unsigned)(cus * per_cu)), dim3(block), lds, s, P);
+        }
         NBL_HIP(hipGetLastError());
         NBL_HIP(hipEventRecord(e1, s));
         NBL_HIP(hipEventSynchronize(e1));
@@ -437,6 +463,16 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         out->boundary_events = cnt[C_BOUND]; out->secondaries = cnt[C_SEC]; out->pending_retries = cnt[C_PEND];
         out->stopped = cnt[C_KILLED];
         out->kernel_ms = ms;
+    });
+}
+
+int nbl_driver_set_sampler(nbl_driver d, nbl_sampler smp)
+{
+    return guarded([&] {
+        require(d != nullptr, "nbl_driver_set_sampler: NULL driver");
+        require(smp == nullptr || sampler_materials(smp) == d->mats,
+            "nbl_driver_set_sampler: sampler was trained for another material set");
+        d->sampler = smp;
     });
 }
 

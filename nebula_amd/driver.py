@@ -152,6 +152,11 @@ class Driver:
         call("nbl_driver_write_slots", self.h, C.byref(v))
 
     # --- persistent batched transport -----------------------------------------
+    def set_sampler(self, sampler):
+        """Learned ICDF sampler for the persistent transport (None = the tables)."""
+        call("nbl_driver_set_sampler", self.h, sampler.h if sampler is not None else None)
+        self.sampler = sampler                 # keep it alive while the driver uses it
+
     def run_persistent(self, beam, block_size=0, blocks_per_cu=0, stack_depth=0, refill_threshold=0,
                        per_pixel=False, hist=False):
         px = np.ascontiguousarray(beam["pixels"], L.PIXEL_DTYPE)

@@ -78,11 +78,12 @@ def add_detector(s: TriangleSoup, z, half):
     s.add_rect((-half, -half, z), (2 * half, 0, 0), (0, 2 * half, 0), 1, 1, MAT_DETECTOR, MAT_DETECTOR, (0, 0, 1))
 
 
-def cube_slab():
-    """Flat Si slab: 1 um cube x,y in [-500,500], z in [-1000,0] (12 triangles),
-    plus a detector plane at z = +10 nm."""
+def cube_slab(material=0):
+    """Flat slab (material index `material`, Si in the workloads): 1 um cube
+    x,y in [-500,500], z in [-1000,0] (12 triangles), plus a detector plane
+    at z = +10 nm."""
     s = TriangleSoup()
-    add_box(s, (-500, -500, -1000), (500, 500, 0), 0, MAT_VACUUM)
+    add_box(s, (-500, -500, -1000), (500, 500, 0), material, MAT_VACUUM)
     add_detector(s, 10.0, 1500.0)
     return s.build()
 
