@@ -66,10 +66,12 @@ def learned_vs_table(primaries, seed, train_samples):
     g = na.Geometry(tris)
     m = na.Materials(specs)
     t0 = time.time()
-    smp = na.Sampler(m, samples=train_samples, e_min=20.0, e_max=1.2 * e_beam, seed=seed)
+    # train over every energy an electron can have in the materials: from the
+    # tables' lower end (1 eV, below every barrier) to above the beam energy
+    smp = na.Sampler(m, samples=train_samples, e_min=0.0, e_max=1.2 * e_beam, seed=seed)
     train_s = time.time() - t0
     rep = smp.report()
-    edges = np.geomspace(30.0, 0.95 * e_beam, 7)
+    edges = np.geomspace(5.0, 0.95 * e_beam, 9)
     ks = {k: [smp.ks_per_bin(kind, mat, edges, n=20000, seed=mat) for mat in range(len(specs))]
           for k, kind in (("elastic", 0), ("inelastic", 1))}
     out = {}
@@ -99,8 +101,10 @@ def learned_vs_table(primaries, seed, train_samples):
             "speedup_learned_vs_table": out["learned"]["primaries_per_s"] / out["table"]["primaries_per_s"],
             "yield_z": z, "yields_equal_3sigma": bool(max(abs(z["se"]), abs(z["bse"])) < 3),
             "ks_per_bin": {"energy_edges_eV": [float(e) for e in edges],
-                           **{k: [[round(float(x), 5) for x in row] for row in v] for k, v in ks.items()},
-                           "max": {k: float(np.max(v)) for k, v in ks.items()}},
+                           "note": "per material (rows) and energy bin; table cells below the Fermi level excluded",
+                           **{k: [[None if not np.isfinite(x) else round(float(x), 5) for x in row] for row in v]
+                              for k, v in ks.items()},
+                           "max": {k: float(np.nanmax(v)) for k, v in ks.items()}},
             "training": {"samples_per_network": rep["samples"], "seconds": train_s,
                          "mse_standardised": {"elastic": rep["loss_elastic"], "inelastic": rep["loss_inelastic"]},
                          "energy_range_eV": [rep["e_min"], rep["e_max"]]}}

@@ -174,7 +174,7 @@ class Sampler:
 
     KINDS = {L.SAMPLER_ELASTIC: L.TAB_ELASTIC_ICDF, L.SAMPLER_INELASTIC: L.TAB_INELASTIC_W0_ICDF}
 
-    def __init__(self, materials: Materials, samples=100_000_000, batch=16384, lr=2e-3, e_min=0.0, e_max=0.0,
+    def __init__(self, materials: Materials, samples=100_000_000, batch=16384, lr=4e-3, e_min=0.0, e_max=0.0,
                  seed=1):
         self.materials = materials              # the tables must outlive the sampler
         o = L.SamplerOpts(samples=int(samples), batch=int(batch), lr=float(lr), e_min=float(e_min),
@@ -220,14 +220,23 @@ class Sampler:
         """KS distance between learned and table samples in each energy bin
         [edges[i], edges[i+1]) (energies log-uniform within a bin, the same
         uniforms for both).  KS is invariant under the monotone map between
-        log omega' and omega', so either may be compared."""
+        log omega' and omega', so either may be compared.  Energies below the
+        material's barrier are left out: an electron there has left the
+        material or stopped (gpu_driver.inl:420-428), so the transport never
+        draws from those cells; a bin entirely below it gives NaN."""
         rng = np.random.default_rng(seed)
+        floor = float(self.materials.prepared[material].spec.barrier)
         out = []
         for lo, hi in zip(edges[:-1], edges[1:]):
+            lo = max(lo, floor)
+            if lo >= hi:
+                out.append(float("nan"))
+                continue
             e = np.exp(rng.uniform(np.log(lo), np.log(hi), n)).astype(np.float32)
             u = ((rng.integers(0, 1 << 24, n) + 1) * 2.0 ** -24).astype(np.float32)
             a, b = self.table(kind, material, e, u), self.eval(kind, material, e, u)
-            out.append(ks_distance(a, b))
+            ok = np.isfinite(a)          # table cells below the Fermi level hold no finite value
+            out.append(ks_distance(a[ok], b[ok]) if ok.sum() >= 100 else float("nan"))
         return out
 
 

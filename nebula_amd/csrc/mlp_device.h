@@ -86,10 +86,10 @@ __device__ __forceinline__ float mlp_value(const dmlp& n, float y)
     return n.kind == MLP_KIND_ELASTIC ? 1.000001f - expf(v) : v;
 }
 
-// Evaluate the network for the 64 samples held by the wavefront's lanes
-// (one per lane; lanes whose sample is not needed pass any finite input).
-// Must be called by all 64 lanes.  Returns this lane's sample's value.
-__device__ __forceinline__ float mlp_eval_wave(const dmlp& n, int mat, float log_e, float u)
+// Evaluate the network for the samples held by lanes 0 .. 16 nblk - 1 (one
+// per lane).  Must be called by all 64 lanes with the same nblk (1..4).
+// Returns this lane's sample's value (defined for lanes < 16 nblk).
+__device__ __forceinline__ float mlp_eval_blocks(const dmlp& n, int mat, float log_e, float u, int nblk)
 {
     const uint32_t lane = __lane_id();
     const uint32_t j = lane & 15u, g = lane >> 4;
@@ -101,7 +101,7 @@ __device__ __forceinline__ float mlp_eval_wave(const dmlp& n, int mat, float log
     // bias loads next to their use (L1/L2 hits) instead of hoisting them all
     // into registers.
 #pragma unroll 1
-    for (int nb = 0; nb < 4; ++nb) {
+    for (int nb = 0; nb < nblk; ++nb) {
         asm volatile("" ::: "memory");
         // B operand of layer 1: features 4g..4g+3 of the sample in lane j + 16 nb
         // (g = 0: the scalars, g = 1, 2: embedding halves, g = 3: zeros)
@@ -168,6 +168,33 @@ __device__ __forceinline__ float mlp_eval_wave(const dmlp& n, int mat, float log
         if (g == (uint32_t)nb) mine = s;
     }
     return mlp_value(n, mine + n.b4);
+}
+
+// All 64 lanes' samples (lanes whose sample is not needed pass any finite input).
+__device__ __forceinline__ float mlp_eval_wave(const dmlp& n, int mat, float log_e, float u)
+{
+    return mlp_eval_blocks(n, mat, log_e, u, 4);
+}
+
+// Only the samples of lanes with need == true: they are moved to the first
+// lanes (a stable partition by one permute), the network runs over
+// ceil(count / 16) blocks, and each lane reads its value back.  Must be
+// called by all 64 lanes; returns the value for lanes with need == true.
+__device__ __forceinline__ float mlp_eval_needed(const dmlp& n, bool need, int mat, float log_e, float u)
+{
+    const uint64_t m = __ballot(need);
+    const uint32_t cnt = (uint32_t)__popcll(m);
+    if (cnt == 0) return 0.0f;
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    const uint32_t lane = __lane_id();
+    // slot of this lane's sample: needed samples first, in lane order
+    const int slot = (int)(need ? below : cnt + (lane - below));
+    const int addr = slot * 4;
+    const float le = __int_as_float(__builtin_amdgcn_ds_permute(addr, __float_as_int(log_e)));
+    const float uu = __int_as_float(__builtin_amdgcn_ds_permute(addr, __float_as_int(u)));
+    const int mm = __builtin_amdgcn_ds_permute(addr, mat);
+    const float v = mlp_eval_blocks(n, mm, le, uu, (int)((cnt + 15) / 16));
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
 }
 
 }  // namespace nbl

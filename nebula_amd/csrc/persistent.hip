@@ -290,8 +290,9 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
                 float y = 0.5f, le = 0.0f;
                 if (el || go) { y = rng.unit(); le = logf(p.e); }
                 float v = 0;
-                if (__ballot(el)) { const float ve = mlp_eval_wave(P.mlp[MLP_KIND_ELASTIC], mat, le, y); if (el) v = ve; }
-                if (__ballot(go)) { const float vi = mlp_eval_wave(P.mlp[MLP_KIND_INELASTIC], mat, le, y); if (go) v = vi; }
+                const float ve = mlp_eval_needed(P.mlp[MLP_KIND_ELASTIC], el, mat, le, y);
+                const float vi = mlp_eval_needed(P.mlp[MLP_KIND_INELASTIC], go, mat, le, y);
+                v = el ? ve : vi;
                 if (el || go) {
                     pend = false;
                     last = NO_TRI;

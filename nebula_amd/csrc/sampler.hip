@@ -128,7 +128,10 @@ __global__ __launch_bounds__(256) void k_out(const float* __restrict__ prm, cons
     if (b < B) {
         float y = w[MLP_H];
         for (int n = 0; n < MLP_H; ++n) y += w[n] * H3[(size_t)n * B + b];
-        const float d = y - T[b];
+        // table cells with no finite value (e.g. log of 0 below the Fermi
+        // level, energies no electron reaches) carry no loss
+        const float t = T[b];
+        const float d = isfinite(t) ? y - t : 0.0f;
         dY[b] = 2.0f * d / (float)B;
         l = (double)d * d;
     }
@@ -273,16 +276,22 @@ __global__ void k_pack(const float* __restrict__ prm, bf16x4* __restrict__ out)
     out[i] = pack4(W[row * K + col], W[row * K + col + 1], W[row * K + col + 2], W[row * K + col + 3]);
 }
 
-// Batch evaluation (all lanes of a wavefront evaluate together).
+// Batch evaluation through the transport's path (mlp_eval_needed): each
+// wavefront evaluates its odd and its even lanes' samples in two compacted
+// passes, so every value goes through the lane permutation the persistent
+// kernel uses.
 __global__ __launch_bounds__(256) void k_eval(dmlp net, const int* __restrict__ mat, const float* __restrict__ energy,
     const float* __restrict__ u, uint32_t n, float* __restrict__ out)
 {
     for (uint32_t w0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); w0 < n; w0 += gridDim.x * blockDim.x) {
         const uint32_t i = w0 + (threadIdx.x & 63u);
-        const bool on = i < n;
+        const bool on = i < n, odd = (threadIdx.x & 1u) != 0;
         const uint32_t k = on ? i : w0;
-        const float v = mlp_eval_wave(net, mat[k], logf(energy[k]), u[k]);
-        if (on) out[i] = v;
+        const int m = mat[k];
+        const float le = logf(energy[k]), uu = u[k];
+        const float v1 = mlp_eval_needed(net, on && odd, m, le, uu);
+        const float v0 = mlp_eval_needed(net, on && !odd, m, le, uu);
+        if (on) out[i] = odd ? v1 : v0;
     }
 }
 
@@ -366,8 +375,11 @@ void train_network(nbl_sampler_s& S, int kind, const nbl_sampler_opts& o, float 
         std::vector<float> t(B);
         T.download(t.data(), B);
         double s1 = 0, s2 = 0;
-        for (float x : t) { s1 += x; s2 += (double)x * x; }
-        const double mean = s1 / B, var = std::max(s2 / B - mean * mean, 1e-12);
+        size_t cnt = 0;
+        for (float x : t)
+            if (std::isfinite(x)) { s1 += x; s2 += (double)x * x; ++cnt; }
+        require(cnt > B / 2, "nbl_sampler_create: the tables have no finite values over most of the energy range");
+        const double mean = s1 / cnt, var = std::max(s2 / cnt - mean * mean, 1e-12);
         a.y_shift = (float)mean;
         a.y_scale = (float)std::sqrt(var);
     }
@@ -423,7 +435,7 @@ int nbl_sampler_create(nbl_materials m, const nbl_sampler_opts* opts, nbl_sample
         require(m && out, "nbl_sampler_create: NULL argument");
         *out = nullptr;
         nbl_sampler_opts o{};
-        o.samples = 100000000ull; o.batch = 16384; o.lr = 2e-3f; o.e_min = 0; o.e_max = 0; o.seed = 1;
+        o.samples = 100000000ull; o.batch = 16384; o.lr = 4e-3f; o.e_min = 0; o.e_max = 0; o.seed = 1;
         if (opts) {
             if (opts->samples) o.samples = opts->samples;
             if (opts->batch) o.batch = opts->batch;

@@ -19,7 +19,7 @@ from nebula_amd.driver import Driver, raster_scan
 
 pytestmark = pytest.mark.gpu
 
-E_LO, E_HI = 20.0, 600.0
+E_LO, E_HI = 1.0, 600.0
 
 
 def test_matrix_core_layout_known_answer():
@@ -72,12 +72,13 @@ def test_network_matches_numpy_bf16_forward(trained, kind):
     m, s = trained
     rng = np.random.default_rng(kind)
     n = 5000                                   # not a multiple of 64: a ragged last wavefront
-    e = np.exp(rng.uniform(np.log(E_LO), np.log(E_HI), n)).astype(np.float32)
+    e = np.exp(rng.uniform(np.log(2.0), np.log(E_HI), n)).astype(np.float32)
     u = ((rng.integers(0, 1 << 24, n) + 1) * 2.0 ** -24).astype(np.float32)
     p, c = s.params(kind)
     for mat in (0, 5):
         got = s.eval(kind, mat, e, u)
         ref = _numpy_forward(p, c, kind, mat, e, u)
+        assert np.all(np.isfinite(got))
         scale = float(np.std(ref))
         err = np.abs(got - ref)
         print(f"  mat {mat}: max |gpu - numpy| {err.max():.3g} (spread {scale:.3g})")
@@ -85,18 +86,20 @@ def test_network_matches_numpy_bf16_forward(trained, kind):
         assert np.quantile(err, 0.999) < 1e-3 * max(scale, 1.0) and err.max() < 2e-2 * max(scale, 1.0)
 
 
-@pytest.mark.parametrize("kind", [L.SAMPLER_ELASTIC, L.SAMPLER_INELASTIC])
-def test_ks_per_energy_bin(trained, kind):
-    """The learned distribution vs the table's, per energy bin and material."""
+@pytest.mark.parametrize("kind,bound", [(L.SAMPLER_ELASTIC, 0.03), (L.SAMPLER_INELASTIC, 0.08)])
+def test_ks_per_energy_bin(trained, kind, bound):
+    """The learned distribution vs the table's, per energy bin (from each
+    material's barrier up) and material.  Measured with this network and
+    budget: <= 0.013 elastic, <= 0.055 inelastic."""
     print(f"[gpu] sampler KS kind={kind}", flush=True)
     m, s = trained
-    edges = np.geomspace(30.0, 550.0, 7)
+    edges = np.geomspace(5.0, 550.0, 9)
     worst = 0.0
     for mat in range(8):
         ks = s.ks_per_bin(kind, mat, edges, n=20000, seed=mat)
         print(f"  mat {mat}: " + " ".join(f"{k:.4f}" for k in ks))
-        worst = max(worst, max(ks))
-    assert worst < 0.05
+        worst = max(worst, np.nanmax(ks))
+    assert worst < bound
 
 
 def test_persistent_learned_vs_table_yields(trained):
