@@ -347,6 +347,38 @@ typedef struct {
 int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch,
     const nbl_persistent_opts* opts, nbl_tallies* out);
 
+/* ---- file formats (io/load_tri_file.cpp, io/load_pri_file.cpp), host only ---- */
+typedef struct { int32_t x, y; } nbl_int2;     /* int2 of io/load_pri_file.h:10 (pixel index) */
+
+/* load_tri_file (load_tri_file.cpp:8-60): text, one triangle per line with 11
+ * columns "material_in material_out Ax Ay Az Bx By Bz Cx Cy Cz" (nm); blank
+ * lines and lines starting with '#' are skipped.  Two calls: with out == NULL
+ * only *n is set.  A file that cannot be opened gives *n = 0 (the reference
+ * returns an empty list); a line with another column count is an
+ * NBL_ERR_RUNTIME error naming the line. */
+int nbl_tri_file_load(const char* path, nbl_triangle* out, uint32_t capacity, uint32_t* n);
+
+/* Primaries that load_pri_file (load_pri_file.cpp:122-154) warns about; they are kept. */
+typedef struct {
+    uint64_t outside_geometry;   /* position not strictly inside [aabb_min, aabb_max] */
+    uint64_t low_energy;         /* kin_energy <= EPSILON                              */
+    uint64_t high_energy;        /* kin_energy > max_energy                             */
+    uint64_t bad_direction;      /* every |dir component| <= EPSILON                    */
+} nbl_pri_warnings;
+
+/* load_pri_file (load_pri_file.cpp:72-187): binary records of 36 bytes,
+ * 7 float32 (pos, dir, energy) + 2 int32 (pixel x, y).  Two calls: with out,
+ * pixels and warnings all NULL only *n is set.  A file that cannot be opened
+ * gives *n = 0; a size that is not a multiple of 36 B is an error. */
+int nbl_pri_file_load(const char* path, nbl_vec3 aabb_min, nbl_vec3 aabb_max, float max_energy,
+    nbl_particle* out, nbl_int2* pixels, uint64_t capacity, uint64_t* n, nbl_pri_warnings* warnings);
+/* sort_pri_file (load_pri_file.cpp:189-229): in place, by the Morton code of
+ * the positions quantised to 16 bits per axis over their bounding box. */
+int nbl_pri_sort_morton(nbl_particle* particles, nbl_int2* pixels, uint64_t n);
+/* prescan_shuffle (load_pri_file.cpp:231-243): in place, the first
+ * prescan_size places get a uniformly random pick of the rest. */
+int nbl_pri_prescan_shuffle(nbl_particle* particles, nbl_int2* pixels, uint64_t n, uint64_t prescan_size);
+
 /* ---- learned ICDF sampler (no reference counterpart: stands in for the
  * elastic cos(theta) table of kieft_elastic, physics/kieft/elastic.h:67-87,
  * and the inelastic omega' table of kieft_inelastic, inelastic.h:68-72) ---- */

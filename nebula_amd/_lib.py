@@ -128,6 +128,18 @@ class PersistentOpts(C.Structure):
                 ("overflow_depth", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
+class Int2(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32)]
+
+
+class PriWarnings(C.Structure):
+    _fields_ = [("outside_geometry", C.c_uint64), ("low_energy", C.c_uint64), ("high_energy", C.c_uint64),
+                ("bad_direction", C.c_uint64)]
+
+
+INT2_DTYPE = np.dtype([("x", np.int32), ("y", np.int32)])
+
+
 class SamplerOpts(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("batch", C.c_uint32), ("lr", C.c_float),
                 ("e_min", C.c_float), ("e_max", C.c_float), ("seed", C.c_uint64),
@@ -187,6 +199,10 @@ SIGNATURES = {
     "nbl_rng_seed": [_U64, _U64, _U32, _P],
     "nbl_rng_uniform_batch": [_P, _U32, _U32, _P],
     "nbl_driver_run_persistent": [_P, C.POINTER(PixelBatch), C.POINTER(PersistentOpts), C.POINTER(Tallies)],
+    "nbl_tri_file_load": [C.c_char_p, _P, _U32, C.POINTER(_U32)],
+    "nbl_pri_file_load": [C.c_char_p, Vec3, Vec3, _F, _P, _P, _U64, C.POINTER(_U64), C.POINTER(PriWarnings)],
+    "nbl_pri_sort_morton": [_P, _P, _U64],
+    "nbl_pri_prescan_shuffle": [_P, _P, _U64, _U64],
     "nbl_sampler_create": [_P, C.POINTER(SamplerOpts), C.POINTER(_P)],
     "nbl_sampler_destroy": [_P],
     "nbl_sampler_get_report": [_P, C.POINTER(SamplerReport)],

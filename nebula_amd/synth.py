@@ -197,3 +197,18 @@ def raw_material(spec: MaterialSpec) -> RawMaterial:
     if key not in _CACHE:
         _CACHE[key] = RawMaterial(spec)
     return _CACHE[key]
+
+
+def write_material_file(spec: MaterialSpec, path):
+    """The material's raw tables as an .nblmat file (read by the nbl_gpu
+    command; layout documented in nebula_amd/csrc/nbl_gpu.cpp)."""
+    raw = raw_material(spec)
+    s = spec
+    with open(path, "wb") as f:
+        f.write(b"NBLMAT1\0")
+        np.array([N_K, N_P, len(raw.outer), 0], "<u4").tofile(f)
+        np.array([K_MIN, K_MAX, P_MIN, P_MAX, s.atomic_mass / AVOGADRO], "<f8").tofile(f)
+        np.array([s.barrier, s.fermi, s.band_gap, s.phonon_loss], "<f4").tofile(f)
+        for a in (raw.el_imfp, raw.el_cos, raw.in_imfp, raw.in_w0, raw.binding):
+            np.ascontiguousarray(a, "<f4").tofile(f)
+        np.ascontiguousarray(raw.outer, "<f8").tofile(f)

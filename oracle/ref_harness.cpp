@@ -8,7 +8,8 @@
  *   geometry/octree/octree_builder.cpp + tribox.cc (leaf sets, Morton order),
  *   geometry/octree.inl (in_domain, propagate),
  *   geometry/trilist.inl (brute-force propagate),
- *   common/util/table_1D.inl, table_2D.inl (get, get_rounddown)
+ *   common/util/table_1D.inl, table_2D.inl (get, get_rounddown),
+ *   io/load_tri_file.cpp, io/load_pri_file.cpp (file formats, primary order)
  * bit for bit.  The physics headers are not included: they pull in the HDF5
  * material loader (io/hdf5_file.h), an external library.
  *
@@ -30,6 +31,8 @@
 #include "geometry/trilist.h"
 #include "common/util/table_1D.h"
 #include "common/util/table_2D.h"
+#include "io/load_tri_file.h"
+#include "io/load_pri_file.h"
 
 #include "../include/nbl_hip.h"
 
@@ -229,6 +232,38 @@ void ref_table2d_get_batch(int32_t w, int32_t h, float x_min, float x_max, float
     auto t = nbl::util::table_2D<float, false>::create(x_min, x_max, w, y_min, y_max, h, d.data());
     for (uint32_t i = 0; i < n; ++i) out[i] = rounddown ? t.get_rounddown(x[i], y[i]) : t.get(x[i], y[i]);
     nbl::util::table_2D<float, false>::destroy(t);
+}
+
+/* ---- io/load_tri_file.cpp, io/load_pri_file.cpp ---- */
+/* load_tri_file; returns the count, fills out (if non-NULL) with r0, e1, e2
+ * as the reference's triangle stores them.  -1 if the reference throws. */
+int64_t ref_tri_file_load(const char* path, nbl_triangle* out)
+{
+    try {
+        const std::vector<triangle> t = nbl::load_tri_file(path);
+        // the reference's triangle is layout-identical to nbl_triangle
+        // (material_in, material_out, r0, e1, e2; core/triangle.h:69-76)
+        static_assert(sizeof(triangle) == sizeof(nbl_triangle), "triangle layout");
+        if (out && !t.empty()) std::memcpy(out, t.data(), t.size() * sizeof(nbl_triangle));
+        return (int64_t)t.size();
+    } catch (...) {
+        return -1;
+    }
+}
+
+/* load_pri_file (+ sort_pri_file if sort, + prescan_shuffle(prescan)); returns the count. */
+int64_t ref_pri_file_load(const char* path, const float* lo, const float* hi, float max_e, int sort, uint64_t prescan,
+    nbl_particle* out, int32_t* pixels)
+{
+    auto pr = nbl::load_pri_file(path, vec3{ lo[0], lo[1], lo[2] }, vec3{ hi[0], hi[1], hi[2] }, max_e);
+    if (sort && !pr.first.empty()) nbl::sort_pri_file(pr.first, pr.second);
+    if (prescan) nbl::prescan_shuffle(pr.first, pr.second, prescan);
+    for (size_t i = 0; i < pr.first.size(); ++i) {
+        const particle& p = pr.first[i];
+        if (out) out[i] = { { p.pos.x, p.pos.y, p.pos.z }, { p.dir.x, p.dir.y, p.dir.z }, p.kin_energy };
+        if (pixels) { pixels[2 * i] = pr.second[i].x; pixels[2 * i + 1] = pr.second[i].y; }
+    }
+    return (int64_t)pr.first.size();
 }
 
 }  /* extern "C" */

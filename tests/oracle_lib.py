@@ -77,9 +77,13 @@ def _load_ref():
         "ref_intersect_ray_batch": (None, [_P, _P, _P, _U32, _P]),
         "ref_table1d_get_batch": (None, [_I32, _F, _F, _P, _P, _U32, _P]),
         "ref_table2d_get_batch": (None, [_I32, _I32, _F, _F, _F, _F, _P, _I32, _P, _P, _U32, _P]),
+        "ref_tri_file_load": (C.c_int64, [C.c_char_p, _P]),
+        "ref_pri_file_load": (C.c_int64, [C.c_char_p, _P, _P, _F, C.c_int, C.c_uint64, _P, _P]),
     }
     for name, (res, args) in sig.items():
-        f = getattr(lib, name)
+        f = getattr(lib, name, None)
+        if f is None:
+            return None                      # an older harness build: rebuild with make -C oracle
         f.restype = res
         f.argtypes = args
     return lib

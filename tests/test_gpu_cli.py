@@ -1,0 +1,64 @@
+"""The nbl_gpu command (the reference's nebula_gpu front end: .tri + .pri +
+materials in, 36-byte detection records out, prescan + frame loop through
+the iteration driver) end to end on the GPU, against the persistent kernel
+on the same geometry, materials and beam (yields within 4 sigma)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import runs
+import nebula_amd as na
+from nebula_amd import _lib as L
+from nebula_amd import files, scenes, synth
+from nebula_amd.driver import Driver, raster_scan
+
+pytestmark = pytest.mark.gpu
+
+CLI = os.path.join(os.path.dirname(L.LIB_PATH), "nbl_gpu")
+
+
+def test_cli_runs_and_matches_persistent_yields(tmp_path):
+    print("[gpu] nbl_gpu end to end", flush=True)
+    tris = scenes.cube_slab()
+    a = tris["r0"].astype(np.float64)
+    files.write_tri_file(tmp_path / "g.tri", tris["material_in"], tris["material_out"],
+                         a, a + tris["e1"], a + tris["e2"], comment="flat Si slab + detector")
+    n = 20_000
+    prim = runs.pencil_primaries(n, 1000.0)
+    pix = np.stack([np.arange(n) % 100, np.arange(n) // 100], axis=1).astype(np.int32)
+    files.write_pri_file(tmp_path / "p.pri", prim, pix)
+    synth.write_material_file(synth.SI, tmp_path / "si.nblmat")
+    r = subprocess.run([CLI, "--capacity", "200000", "--prescan-size=1000", "--seed", "7",
+                        str(tmp_path / "g.tri"), str(tmp_path / "p.pri"), str(tmp_path / "si.nblmat")],
+                       capture_output=True, timeout=300)
+    err = r.stderr.decode(errors="replace")
+    print(err[-600:])
+    assert r.returncode == 0, err
+    rec = files.read_records(r.stdout)
+    assert len(r.stdout) == 36 * len(rec) and len(rec) > 0
+    assert np.all((rec["pixel"][:, 0] >= 0) & (rec["pixel"][:, 0] < 100))
+    assert np.all((rec["pixel"][:, 1] >= 0) & (rec["pixel"][:, 1] < n // 100))
+    assert np.all(np.abs(rec["pos"][:, 2] - 10.0) < 1e-3)      # all detected on the detector plane
+    se, bse = int((rec["kin_energy"] < 50).sum()), int((rec["kin_energy"] >= 50).sum())
+    g = na.Geometry(tris); m = na.Materials([synth.SI])
+    with Driver(g, m, capacity=1 << 16, seed=3) as drv:
+        t = drv.run_persistent(raster_scan(np.zeros(1), np.zeros(1), 5.0, 1000.0, 200_000, 0.0, seed=5))
+    for k_cli, k_p in ((se, t["detected_se"]), (bse, t["detected_bse"])):
+        z = runs.yield_z(k_cli, n, k_p, t["primaries"])
+        print(f"  cli {k_cli / n:.4f} persistent {k_p / t['primaries']:.4f} z {z:+.2f}")
+        assert abs(z) < 4
+
+
+def test_cli_usage_and_errors(tmp_path):
+    r = subprocess.run([CLI], capture_output=True, timeout=60)
+    assert r.returncode == 1 and b"Usage" in r.stderr
+    r = subprocess.run([CLI, "--bogus", "1", "a", "b", "c"], capture_output=True, timeout=60)
+    assert r.returncode == 1 and b"Unexpected key 'bogus'" in r.stderr
+    (tmp_path / "g.tri").write_text("0 -123 0 0 0 1 0 0 0 1 0\n2 -123 0 0 0 1 0 0 0 1 0\n")
+    (tmp_path / "p.pri").write_bytes(b"")
+    synth.write_material_file(synth.SI, tmp_path / "si.nblmat")
+    r = subprocess.run([CLI, str(tmp_path / "g.tri"), str(tmp_path / "p.pri"), str(tmp_path / "si.nblmat")],
+                       capture_output=True, timeout=120)
+    assert r.returncode == 1 and b"not enough materials" in r.stderr
