@@ -318,6 +318,20 @@ typedef struct {
 } nbl_pixel_batch;
 
 #define NBL_HIST_BINS 1024
+
+/* Energy-deposition tally (the GPU counterpart of energydeposit_cpu_driver,
+ * drivers/cpu/energydeposit_cpu_driver.h:44-94): every elastic or inelastic
+ * event that changes the electron's energy deposits (energy before - energy
+ * after) at the event position; the deposits are summed into a voxel grid
+ * instead of written as 28-byte records (cpu_energydep.cpp:184-192). */
+typedef struct {
+    uint32_t nx, ny, nz;     /* voxels along x, y, z                                  */
+    nbl_vec3 lo, hi;         /* grid box, nm                                          */
+    double* energy;          /* [nz][ny][nx] out: deposited energy per voxel, eV      */
+    double total;            /* out: all deposited energy, eV (inside the box or not) */
+    uint64_t count;          /* out: deposit events                                    */
+} nbl_deposit_grid;
+
 typedef struct {
     uint64_t primaries;      /* primaries whose cascades ended                    */
     uint64_t detected_se;    /* detected with E < 50 eV                            */
@@ -331,6 +345,7 @@ typedef struct {
     uint32_t* pixel_se;      /* optional [n_pixels] out (may be NULL)              */
     uint32_t* pixel_bse;     /* optional [n_pixels] out (may be NULL)              */
     uint32_t* hist;          /* optional [NBL_HIST_BINS] detected-energy histogram, 1-eV bins */
+    nbl_deposit_grid* deposit; /* optional energy-deposition tally (may be NULL)      */
 } nbl_tallies;
 
 typedef struct {

@@ -119,7 +119,13 @@ class Tallies(C.Structure):
                 ("rounddowns", C.c_uint64), ("node_steps", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("boundary_events", C.c_uint64), ("secondaries", C.c_uint64),
                 ("pending_retries", C.c_uint64), ("stopped", C.c_uint64), ("kernel_ms", C.c_double),
-                ("pixel_se", C.c_void_p), ("pixel_bse", C.c_void_p), ("hist", C.c_void_p)]
+                ("pixel_se", C.c_void_p), ("pixel_bse", C.c_void_p), ("hist", C.c_void_p),
+                ("deposit", C.c_void_p)]
+
+
+class DepositGrid(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("lo", Vec3), ("hi", Vec3),
+                ("energy", C.c_void_p), ("total", C.c_double), ("count", C.c_uint64)]
 
 
 class PersistentOpts(C.Structure):

@@ -31,9 +31,10 @@ using namespace nbl;
 namespace {
 
 constexpr uint32_t STEP_CAP = 1u << 20;
-constexpr int N_COUNTERS = 14;
+constexpr int N_COUNTERS = 15;
 // counter slots
-enum { C_PRIM = 0, C_SE, C_BSE, C_EVENTS, C_L1D, C_L2D, C_RD, C_NODES, C_TRIS, C_BOUND, C_SEC, C_PEND, C_KILLED, C_DET };
+enum { C_PRIM = 0, C_SE, C_BSE, C_EVENTS, C_L1D, C_L2D, C_RD, C_NODES, C_TRIS, C_BOUND, C_SEC, C_PEND, C_KILLED, C_DET,
+       C_DEPN };
 
 constexpr uint32_t N_QUEUES = 8;       // one primary queue per XCD (blocks go round-robin over XCDs)
 constexpr uint32_t QSTRIDE = 16;       // queue counters 128 B apart
@@ -66,6 +67,11 @@ struct pers_params {
     uint32_t ovf;                       // HBM stack entries per wavefront
     uint32_t refill_threshold;
     dmlp mlp[2];                        // learned sampler (elastic, inelastic) when LEARNED
+    // energy-deposition tally (dep == nullptr: off)
+    double* dep;                        // [dnz][dny][dnx] voxels
+    double* dep_total;                  // all deposits
+    uint32_t dnx, dny, dnz;
+    f3 dlo, dinv;                       // voxel index = floor((p - dlo) * dinv)
 };
 
 // One logical stack per wavefront: entries [0, D) in LDS, [D, D + G) in HBM.
@@ -141,7 +147,9 @@ __device__ void make_primary(const pers_params& P, uint64_t i, dparticle& p, uin
 // sampler's networks (evaluated by the whole wavefront) instead of the tables.
 // The learned variant needs ~150 registers while the networks run, so it is
 // built for 12 wavefronts per CU (3 per SIMD) instead of 16.
-template<bool LEARNED>
+// DEPOSIT: the energy-deposition tally is on (a separate instantiation, so
+// that the plain kernel carries none of its registers).
+template<bool LEARNED, bool DEPOSIT>
 __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params P)
 {
     extern __shared__ float lds[];
@@ -172,7 +180,8 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
     // (population counts of ballots, held in scalar registers).
     prop_counters walked{ 0, 0 };
     uint32_t n_prims = 0, n_events = 0, n_l1d = 0, n_l2d = 0, n_rd = 0, n_bound = 0, n_sec = 0, n_pend = 0,
-             n_se = 0, n_bse = 0, n_killed = 0;
+             n_se = 0, n_bse = 0, n_killed = 0, n_dep = 0;
+    double dep_sum = 0;                               // this lane's deposited energy
 
     for (;;) {
         // ---- refill idle lanes: stack first, then fresh primaries ----------
@@ -284,6 +293,7 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
         {
             // elastic and inelastic lanes share one pass through each part
             const bool el = alive && ev == EV_ELASTIC;
+            const float e_before = p.e;
             bool rot = false;
             scatter_state ss;
             if (LEARNED) {
@@ -306,6 +316,20 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
             f3 nd{ 0, 0, 0 };
             if (rot) nd = scatter_frame(p, rng);
             if (rot) made = scatter_end(P.mats[mat], p, rng, ss, nd, &q);
+            // energy deposit of a scatter event (energydeposit_cpu_driver.h:72-94)
+            if (DEPOSIT) {
+                const float de = (el || go) ? e_before - p.e : 0.0f;
+                const bool dep = de != 0.0f;
+                n_dep += __popcll(__ballot(dep));
+                if (dep) {
+                    dep_sum += de;
+                    const int ix = (int)floorf((p.pos.x - P.dlo.x) * P.dinv.x);
+                    const int iy = (int)floorf((p.pos.y - P.dlo.y) * P.dinv.y);
+                    const int iz = (int)floorf((p.pos.z - P.dlo.z) * P.dinv.z);
+                    if (ix >= 0 && iy >= 0 && iz >= 0 && ix < (int)P.dnx && iy < (int)P.dny && iz < (int)P.dnz)
+                        atomicAdd(&P.dep[((size_t)iz * P.dny + iy) * P.dnx + ix], (double)de);
+                }
+            }
         }
         n_se += __popcll(__ballot(se));
         n_bse += __popcll(__ballot(bse));
@@ -324,7 +348,7 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
 
     // ---- counters: one atomic per wavefront per counter ----------------------
     const uint32_t wave_vals[N_COUNTERS] = { n_prims, n_se, n_bse, n_events, n_l1d, n_l2d, n_rd, 0, 0, n_bound,
-                                             n_sec, n_pend, n_killed, n_se + n_bse };
+                                             n_sec, n_pend, n_killed, n_se + n_bse, n_dep };
     for (int k = 0; k < N_COUNTERS; ++k) {
         uint64_t v = wave_vals[k];
         if (k == C_NODES || k == C_TRIS) {
@@ -332,6 +356,10 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         }
         if (lane == 0 && v) atomicAdd(&P.counters[k], (unsigned long long)v);
+    }
+    if (DEPOSIT) {
+        for (int off = 32; off > 0; off >>= 1) dep_sum += __shfl_xor(dep_sum, off);
+        if (lane == 0 && dep_sum != 0) atomicAdd(P.dep_total, dep_sum);
     }
 }
 
@@ -432,16 +460,39 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         if (d->overflow.n < ovf_floats) d->overflow.alloc(ovf_floats);
         P.overflow = d->overflow.p;
         P.depth = depth; P.ovf = ovf; P.refill_threshold = thresh;
+        // energy-deposition tally
+        nbl_deposit_grid* dg = out->deposit;
+        dbuf<double> ddep, dtot;
+        P.dep = nullptr; P.dep_total = nullptr;
+        if (dg) {
+            require(dg->energy != nullptr && dg->nx > 0 && dg->ny > 0 && dg->nz > 0,
+                "nbl_driver_run_persistent: deposit grid needs nx, ny, nz > 0 and an energy array");
+            require((uint64_t)dg->nx * dg->ny * dg->nz <= (1ull << 30), "nbl_driver_run_persistent: deposit grid too large");
+            require(dg->hi.x > dg->lo.x && dg->hi.y > dg->lo.y && dg->hi.z > dg->lo.z,
+                "nbl_driver_run_persistent: deposit grid box is empty");
+            const size_t nv = (size_t)dg->nx * dg->ny * dg->nz;
+            ddep.alloc(nv); dtot.alloc(1);
+            NBL_HIP(hipMemsetAsync(ddep.p, 0, nv * sizeof(double), s));
+            NBL_HIP(hipMemsetAsync(dtot.p, 0, sizeof(double), s));
+            P.dep = ddep.p; P.dep_total = dtot.p;
+            P.dnx = dg->nx; P.dny = dg->ny; P.dnz = dg->nz;
+            P.dlo = { dg->lo.x, dg->lo.y, dg->lo.z };
+            P.dinv = { (float)dg->nx / (dg->hi.x - dg->lo.x), (float)dg->ny / (dg->hi.y - dg->lo.y),
+                       (float)dg->nz / (dg->hi.z - dg->lo.z) };
+        }
 
         hipEvent_t e0, e1;
         NBL_HIP(hipEventCreate(&e0)); NBL_HIP(hipEventCreate(&e1));
         NBL_HIP(hipEventRecord(e0, s));
+        const dim3 grid((unsigned)(cus * per_cu)), threads_per_block(block);
         if (d->sampler) {
             const dmlp* v = sampler_views(d->sampler);
             P.mlp[0] = v[0]; P.mlp[1] = v[1];
-            hipLaunchKernelGGL(k_persistent<true>, dim3((unsigned)(cus * per_cu)), dim3(block), lds, s, P);
+            if (dg) hipLaunchKernelGGL((k_persistent<true, true>), grid, threads_per_block, lds, s, P);
+            else hipLaunchKernelGGL((k_persistent<true, false>), grid, threads_per_block, lds, s, P);
         } else {
-            hipLaunchKernelGGL(k_persistent<false>, dim3((unsigned)(cus * per_cu)), dim3(block), lds, s, P);
+            if (dg) hipLaunchKernelGGL((k_persistent<false, true>), grid, threads_per_block, lds, s, P);
+            else hipLaunchKernelGGL((k_persistent<false, false>), grid, threads_per_block, lds, s, P);
         }
         NBL_HIP(hipGetLastError());
         NBL_HIP(hipEventRecord(e1, s));
@@ -456,6 +507,10 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         dhist.download(hist.data(), NBL_HIST_BINS, s);
         if (out->pixel_se) dse.download(out->pixel_se, np, s);
         if (out->pixel_bse) dbse.download(out->pixel_bse, np, s);
+        if (dg) {
+            ddep.download(dg->energy, (size_t)dg->nx * dg->ny * dg->nz, s);
+            dtot.download(&dg->total, 1, s);
+        }
         NBL_HIP(hipStreamSynchronize(s));
         if (out->hist) for (int b = 0; b < NBL_HIST_BINS; ++b) out->hist[b] = (uint32_t)hist[b];
         out->primaries = cnt[C_PRIM]; out->detected_se = cnt[C_SE]; out->detected_bse = cnt[C_BSE];
@@ -463,6 +518,7 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         out->rounddowns = cnt[C_RD]; out->node_steps = cnt[C_NODES]; out->tri_tests = cnt[C_TRIS];
         out->boundary_events = cnt[C_BOUND]; out->secondaries = cnt[C_SEC]; out->pending_retries = cnt[C_PEND];
         out->stopped = cnt[C_KILLED];
+        if (dg) dg->count = cnt[C_DEPN];
         out->kernel_ms = ms;
     });
 }

@@ -158,7 +158,11 @@ class Driver:
         self.sampler = sampler                 # keep it alive while the driver uses it
 
     def run_persistent(self, beam, block_size=0, blocks_per_cu=0, stack_depth=0, refill_threshold=0,
-                       per_pixel=False, hist=False):
+                       per_pixel=False, hist=False, deposit=None):
+        """One persistent launch over the pixel batch `beam`.  deposit =
+        dict(shape=(nx, ny, nz), lo=(x, y, z), hi=(x, y, z)) turns on the
+        energy-deposition tally: out["deposit"] is the [nz, ny, nx] grid (eV),
+        out["deposit_total"] and out["deposit_count"] cover every deposit."""
         px = np.ascontiguousarray(beam["pixels"], L.PIXEL_DTYPE)
         d = beam.get("dir", (0.0, 0.0, -1.0))
         b = L.PixelBatch(pixels=ptr(px), n_pixels=len(px), dir=L.Vec3(*d), seed=int(beam.get("seed", 0)))
@@ -169,8 +173,18 @@ class Driver:
         bse = np.zeros(len(px), np.uint32) if per_pixel else None
         h = np.zeros(L.HIST_BINS, np.uint32) if hist else None
         t.pixel_se, t.pixel_bse, t.hist = ptr(se), ptr(bse), ptr(h)
+        dep = grid = None
+        if deposit is not None:
+            nx, ny, nz = (int(v) for v in deposit["shape"])
+            grid = np.zeros((nz, ny, nx), np.float64)
+            dep = L.DepositGrid(nx=nx, ny=ny, nz=nz, lo=L.Vec3(*map(float, deposit["lo"])),
+                                hi=L.Vec3(*map(float, deposit["hi"])), energy=ptr(grid))
+            t.deposit = C.cast(C.pointer(dep), C.c_void_p)
         call("nbl_driver_run_persistent", self.h, C.byref(b), C.byref(o), C.byref(t))
-        out = {f: getattr(t, f) for f, _ in L.Tallies._fields_ if f not in ("pixel_se", "pixel_bse", "hist")}
+        out = {f: getattr(t, f) for f, _ in L.Tallies._fields_
+               if f not in ("pixel_se", "pixel_bse", "hist", "deposit")}
+        if dep is not None:
+            out["deposit"], out["deposit_total"], out["deposit_count"] = grid, dep.total, dep.count
         if per_pixel:
             out["pixel_se"], out["pixel_bse"] = se, bse
         if hist:

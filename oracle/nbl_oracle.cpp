@@ -858,6 +858,26 @@ struct cpu_particle {
 };
 struct cascade { uint32_t next_secondary_tag, running_count; };
 
+/* Energy-deposition tally of energydeposit_cpu_driver (drivers/cpu/
+ * energydeposit_cpu_driver.h:44-94): a scatter event whose particle's
+ * energy changes deposits before.E - after.E at after.pos; here summed
+ * into a voxel grid instead of written as records (cpu_energydep.cpp:184-192). */
+struct deposit_grid {
+    int nx = 0, ny = 0, nz = 0;
+    vec3 lo{}, inv{};                  /* voxel index = floor((p - lo) * inv) */
+    std::vector<double> e;
+    double total = 0;
+    uint64_t count = 0;
+    void add(vec3 p, float de)
+    {
+        total += de; ++count;
+        const int ix = (int)std::floor((p.x - lo.x) * inv.x), iy = (int)std::floor((p.y - lo.y) * inv.y),
+                  iz = (int)std::floor((p.z - lo.z) * inv.z);
+        if (ix < 0 || iy < 0 || iz < 0 || ix >= nx || iy >= ny || iz >= nz) return;
+        e[((size_t)iz * ny + iy) * nx + ix] += de;
+    }
+};
+
 template<typename R>
 struct cpu_driver {
     const geometry& geom; const std::vector<material>& mats;
@@ -866,6 +886,7 @@ struct cpu_driver {
     std::vector<cpu_particle> particles;
     std::map<uint32_t, cascade> cascades;
     counters c;
+    deposit_grid* dep = nullptr;         /* energy-deposition tally, or none */
 
     cpu_driver(const geometry& g, const std::vector<material>& m, float mn, float mx, R r)
         : geom(g), mats(m), min_energy(mn), max_energy(mx), rng(r) {}
@@ -950,11 +971,20 @@ struct cpu_driver {
             particles[idx].data = p;
         }
     }
-    /* simple_cpu_driver::simulate_to_end (simple_cpu_driver.h:54-66) */
+    /* simple_cpu_driver::simulate_to_end (simple_cpu_driver.h:54-66); with a
+     * deposit tally, energydeposit_cpu_driver::simulate_to_end
+     * (energydeposit_cpu_driver.h:72-94): track = next_scatter after init */
     void simulate_to_end()
     {
         for (size_t idx = 0; idx < particles.size(); ++idx)
-            while (active(idx)) { init(idx); intersect(idx); scatter(idx); }
+            while (active(idx)) {
+                init(idx);
+                const bool track = particles[idx].status == CPU_SCATTER;
+                const float before = particles[idx].data.kin_energy;
+                intersect(idx); scatter(idx);
+                if (dep && track && before != particles[idx].data.kin_energy)
+                    dep->add(particles[idx].data.pos, before - particles[idx].data.kin_energy);
+            }
     }
     /* flush_detected + flush_terminated (cpu_driver.inl:59-68, cpu_particle_manager.inl:45-78) */
     template<typename F> void flush_detected(F f)
@@ -1274,10 +1304,44 @@ void orc_slots_iteration(orc_slots o, orc_materials mm, orc_geometry gg, float m
     }
 }
 
+static int mt_run(orc_geometry g, orc_materials m, const nbl_particle* primaries, const uint32_t* tags,
+    uint64_t n, float min_energy, uint32_t seed, int32_t n_threads, int32_t rng_kind, double time_limit_s,
+    nbl_particle* det, uint32_t* det_tags, uint64_t max_det, uint64_t* hist, orc_run_stats* stats,
+    const deposit_grid* dep_spec, deposit_grid* dep_out);
+
 /* cpu_mt_main.cpp:158-219 */
 int orc_cpu_mt_run(orc_geometry g, orc_materials m, const nbl_particle* primaries, const uint32_t* tags,
     uint64_t n, float min_energy, uint32_t seed, int32_t n_threads, int32_t rng_kind, double time_limit_s,
     nbl_particle* det, uint32_t* det_tags, uint64_t max_det, uint64_t* hist, orc_run_stats* stats)
+{
+    return mt_run(g, m, primaries, tags, n, min_energy, seed, n_threads, rng_kind, time_limit_s, det, det_tags,
+        max_det, hist, stats, nullptr, nullptr);
+}
+
+/* cpu_energydep.cpp:158-219: the same run with an energy-deposition tally */
+int orc_cpu_mt_run_deposit(orc_geometry g, orc_materials m, const nbl_particle* primaries, const uint32_t* tags,
+    uint64_t n, float min_energy, uint32_t seed, int32_t n_threads, uint64_t* hist, orc_run_stats* stats,
+    int32_t nx, int32_t ny, int32_t nz, const float* lo, const float* hi, double* energy, double* total,
+    uint64_t* count)
+{
+    deposit_grid spec;
+    spec.nx = nx; spec.ny = ny; spec.nz = nz;
+    spec.lo = { lo[0], lo[1], lo[2] };
+    spec.inv = { (float)nx / (hi[0] - lo[0]), (float)ny / (hi[1] - lo[1]), (float)nz / (hi[2] - lo[2]) };
+    deposit_grid out = spec;
+    out.e.assign((size_t)nx * ny * nz, 0.0);
+    const int r = mt_run(g, m, primaries, tags, n, min_energy, seed, n_threads, ORC_RNG_XORWOW, 0, nullptr, nullptr,
+        0, hist, stats, &spec, &out);
+    if (energy) std::copy(out.e.begin(), out.e.end(), energy);
+    if (total) *total = out.total;
+    if (count) *count = out.count;
+    return r;
+}
+
+static int mt_run(orc_geometry g, orc_materials m, const nbl_particle* primaries, const uint32_t* tags,
+    uint64_t n, float min_energy, uint32_t seed, int32_t n_threads, int32_t rng_kind, double time_limit_s,
+    nbl_particle* det, uint32_t* det_tags, uint64_t max_det, uint64_t* hist, orc_run_stats* stats,
+    const deposit_grid* dep_spec, deposit_grid* dep_out)
 {
     if (n_threads <= 0) n_threads = (int32_t)std::max(1u, std::thread::hardware_concurrency());
     const float max_energy = orc_materials_max_energy(m);
@@ -1293,7 +1357,10 @@ int orc_cpu_mt_run(orc_geometry g, orc_materials m, const nbl_particle* primarie
         counters c; uint64_t prim = 0, d_se = 0, d_bse = 0, d_all = 0;
         std::vector<uint64_t> hl(NBL_HIST_BINS, 0);
         std::vector<std::pair<nbl_particle, uint32_t>> local;
+        deposit_grid dl;
+        if (dep_spec) { dl = *dep_spec; dl.e.assign((size_t)dl.nx * dl.ny * dl.nz, 0.0); }
         auto body = [&](auto& drv) {
+            if (dep_spec) drv.dep = &dl;
             for (;;) {
                 uint64_t i;
                 {   /* work_pool::get_work(1) (work_pool.h:45-56) */
@@ -1335,6 +1402,10 @@ int orc_cpu_mt_run(orc_geometry g, orc_materials m, const nbl_particle* primarie
         tot.node_steps += c.node_steps; tot.tri_tests += c.tri_tests;
         tot.boundary_events += c.boundary_events; tot.secondaries += c.secondaries;
         for (int b = 0; b < NBL_HIST_BINS; ++b) h[b] += hl[b];
+        if (dep_out) {
+            for (size_t k = 0; k < dl.e.size(); ++k) dep_out->e[k] += dl.e[k];
+            dep_out->total += dl.total; dep_out->count += dl.count;
+        }
         for (auto& e : local) {
             if (n_det < max_det) { det[n_det] = e.first; if (det_tags) det_tags[n_det] = e.second; }
             ++n_det;

@@ -92,6 +92,14 @@ int orc_cpu_mt_run(orc_geometry g, orc_materials m, const nbl_particle* primarie
     int32_t n_threads, int32_t rng_kind, double time_limit_s,
     nbl_particle* det, uint32_t* det_tags, uint64_t max_det,
     uint64_t* hist, orc_run_stats* stats);
+/* The same run (XORWOW streams, no time limit) with the energy-deposition
+ * tally of energydeposit_cpu_driver (energydeposit_cpu_driver.h:72-94) summed
+ * into an nx x ny x nz grid over [lo, hi] (energy[(iz*ny + iy)*nx + ix], eV);
+ * *total and *count cover every deposit, inside the grid or not. */
+int orc_cpu_mt_run_deposit(orc_geometry g, orc_materials m, const nbl_particle* primaries,
+    const uint32_t* tags, uint64_t n, float min_energy, uint32_t seed, int32_t n_threads,
+    uint64_t* hist, orc_run_stats* stats, int32_t nx, int32_t ny, int32_t nz, const float* lo,
+    const float* hi, double* energy, double* total, uint64_t* count);
 
 #ifdef __cplusplus
 }

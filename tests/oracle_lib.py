@@ -53,6 +53,8 @@ def _load_orc():
         "orc_sort_status": (None, [_P, _U32, _P]),
         "orc_cpu_mt_run": (C.c_int, [_P, _P, _P, _P, _U64, _F, _U32, _I32, _I32, _D, _P, _P, _U64, _P,
                                      C.POINTER(RunStats)]),
+        "orc_cpu_mt_run_deposit": (C.c_int, [_P, _P, _P, _P, _U64, _F, _U32, _I32, _P, _P, _I32, _I32, _I32,
+                                             _P, _P, _P, _P, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

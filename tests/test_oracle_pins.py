@@ -214,3 +214,33 @@ def test_xorwow_recurrence_and_uniformity():
     # neighbouring sequences are uncorrelated
     a = u.reshape(1000, 200)
     assert abs(np.corrcoef(a[:-1].ravel(), a[1:].ravel())[0, 1]) < 0.01
+
+
+def test_oracle_deposit_tally_invariants():
+    """The oracle's energydeposit_cpu_driver rule: one deposit per elastic or
+    inelastic event that changes the energy (nearly all of them; an elastic
+    event above 200 eV with cos(theta) == 1 exactly loses nothing), deposits
+    positive, the grid a part of the total; a grid covering the whole
+    domain holds all of it."""
+    import ctypes as C
+    from nebula_amd import scenes, synth
+    import nebula_amd.api as api
+    import runs
+    tris = scenes.cube_slab()
+    og = ol.OrcGeometry(tris); om = ol.OrcMaterials([api.prepared(synth.SI).desc])
+    n = 300
+    prim = runs.pencil_primaries(n, 1000.0)
+    tags = np.arange(n, dtype=np.uint32)
+    for shape, lo, hi in (((1, 1, 20), (-500, -500, -100), (500, 500, 0)),
+                          ((2, 2, 2), (-2000, -2000, -2000), (2000, 2000, 2000))):
+        grid = np.zeros(shape[::-1], np.float64)
+        tot = C.c_double(); cnt = C.c_uint64(); st = ol.RunStats()
+        lo = np.asarray(lo, np.float32); hi = np.asarray(hi, np.float32)
+        assert ol.orc.orc_cpu_mt_run_deposit(og.h, om.h, ol.p(prim), ol.p(tags), n, 0.0, 7, 4, None, C.byref(st),
+                                             *shape, ol.p(lo), ol.p(hi), ol.p(grid), C.byref(tot), C.byref(cnt)) == 0
+        assert 0.99 * st.lookups_2d <= cnt.value <= st.lookups_2d and cnt.value > 0
+        assert np.all(grid >= 0)
+        if shape == (2, 2, 2):
+            assert abs(grid.sum() - tot.value) <= 1e-9 * tot.value
+        else:
+            assert grid.sum() <= tot.value
