@@ -71,9 +71,24 @@ typedef struct {
     const float* data;
 } nbl_table2d_desc;
 
-/* A material with Kieft-Bosch elastic + inelastic physics, tables already
- * prepared as the reference's create() functions leave them
- * (physics/kieft/elastic.h:120-166, physics/kieft/inelastic.h:212-327). */
+/* 3D table: value(i,j,k) = data[(i*height + j)*depth + k]; i along x, j along
+ * y, k along z (table_3D.inl:70-74 storage order). */
+typedef struct {
+    int32_t width, height, depth;
+    float x_min, x_max, y_min, y_max, z_min, z_max;
+    const float* data;
+} nbl_table3d_desc;
+
+#define NBL_INELASTIC_KIEFT 0   /* kieft_inelastic (physics/kieft/inelastic.h)            */
+#define NBL_INELASTIC_PENN  1   /* full_penn (physics/full_penn.h), the reference default */
+
+/* A material with Kieft-Bosch elastic physics and Kieft-Bosch or full Penn
+ * inelastic physics, tables already prepared as the reference's create()
+ * functions leave them (physics/kieft/elastic.h:120-166,
+ * physics/kieft/inelastic.h:212-327, physics/full_penn.h:296-399).  For full
+ * Penn, inelastic_log_imfp and inelastic_log_w0_icdf hold its log IMFP and log
+ * omega ICDF (same roles, same prep), penn_q_icdf and penn_ionisation its 3D
+ * tables, and ionisation is unused.  All materials of a set use one model. */
 typedef struct {
     float barrier;      /* E_F + work function, eV (core/material.h:52-57)            */
     float fermi;        /* eV (inelastic.h:220)                                       */
@@ -85,7 +100,28 @@ typedef struct {
     nbl_table1d_desc inelastic_log_imfp;     /* log(imfp/nm^-1) vs log(K/eV)          */
     nbl_table2d_desc inelastic_log_w0_icdf;  /* log(omega'/eV) vs (log K, P)          */
     nbl_table2d_desc ionisation;             /* binding/eV vs (log omega', P), rounddown */
+    int32_t inelastic_model;                 /* NBL_INELASTIC_KIEFT or NBL_INELASTIC_PENN */
+    nbl_table3d_desc penn_q_icdf;            /* hbar q/sqrt(2m) / eV^1/2 vs (log K, omega/K, P) */
+    nbl_table3d_desc penn_ionisation;        /* binding/eV vs (log K, log(omega/K), P), rounddown; -1 = none */
 } nbl_material_desc;
+
+/* Raw full Penn data (the /full_penn and /ionization groups of a material
+ * file) before full_penn::create's post-processing.  Energy axes are
+ * log-spaced, probability and omega/K (q table) axes linear, the omega/K axis
+ * of the binding table log-spaced. */
+typedef struct {
+    float fermi, band_gap;
+    int32_t n_k; double k_min, k_max;        /* IMFP and omega ICDF energy grid, eV   */
+    int32_t n_p; double p_min, p_max;        /* omega ICDF probability grid           */
+    const float* imfp;                       /* [n_k]         nm^-1                   */
+    const float* omega_icdf;                 /* [n_k][n_p]    eV                      */
+    int32_t q_nk, q_nw, q_np;
+    double q_k_min, q_k_max, q_w_min, q_w_max, q_p_min, q_p_max;
+    const float* q_icdf;                     /* [q_nk][q_nw][q_np]  q in nm^-1        */
+    int32_t i_nk, i_nw, i_np;
+    double i_k_min, i_k_max, i_w_min, i_w_max, i_p_min, i_p_max;
+    const float* binding_icdf;               /* [i_nk][i_nw][i_np]  eV                */
+} nbl_penn_raw;
 
 /* Raw cross-section data as a cross-section tool writes it, before the
  * reference's create() post-processing.  All energy grids are log-spaced
@@ -179,6 +215,12 @@ int nbl_kieft_prepare(const nbl_kieft_raw* raw,
 /* gpu_material_manager::create(cpu_material_manager const&)
  * (gpu_material_manager.inl:5-26): uploads n materials. */
 int nbl_materials_create(const nbl_material_desc* mats, int32_t n, nbl_materials* out);
+/* full_penn::create (physics/full_penn.h:296-399): prepares the full Penn
+ * tables into the caller's arrays (log_imfp [n_k], log_omega_icdf [n_k*n_p],
+ * q_icdf [q_nk*q_nw*q_np], ionisation [i_nk*i_nw*i_np]) and switches d
+ * (elastic part already filled by nbl_kieft_prepare) to the full Penn model. */
+int nbl_penn_prepare(const nbl_penn_raw* raw, float* log_imfp, float* log_omega_icdf, float* q_icdf,
+    float* ionisation, nbl_material_desc* d);
 int nbl_materials_destroy(nbl_materials m);
 /* Maximal energy accepted by all materials (cpu_material_manager::get_max_energy,
  * scatter_list.h:64-69, elastic.h:46-49). */
@@ -197,12 +239,20 @@ int nbl_table1d_get_batch(nbl_materials m, int32_t material, int32_t table,
  * table_2D::get_rounddown (table_2D.inl:96-104), batched on the device. */
 int nbl_table2d_get_batch(nbl_materials m, int32_t material, int32_t table,
     int32_t rounddown, const float* x, const float* y, uint32_t n, float* out);
+#define NBL_TAB_PENN_Q_ICDF       5 /* 3D */
+#define NBL_TAB_PENN_IONISATION   6 /* 3D */
+/* table_3D::get (table_3D.inl:78-106) or, with rounddown != 0,
+ * table_3D::get_rounddown (table_3D.inl:108-118), batched on the device
+ * (full Penn materials only). */
+int nbl_table3d_get_batch(nbl_materials m, int32_t material, int32_t table,
+    int32_t rounddown, const float* x, const float* y, const float* z, uint32_t n, float* out);
 
 /* ---- single-event physics with injected uniforms (unit-parity entry points) */
 /* Event kinds. */
 #define NBL_EVENT_SAMPLE_PATH 0 /* material::sample_path (scatter_list.h:106-112)        */
 #define NBL_EVENT_ELASTIC     1 /* kieft_elastic::execute (elastic.h:67-115)              */
-#define NBL_EVENT_INELASTIC   2 /* kieft_inelastic::execute (inelastic.h:68-206)          */
+#define NBL_EVENT_INELASTIC   2 /* kieft_inelastic::execute (inelastic.h:68-206), or
+                                   full_penn::execute (full_penn.h:66-288) for full Penn materials */
 #define NBL_EVENT_INTERSECT   3 /* boundary_intersect::execute (boundary_intersect.h:39-156) */
 /* Outcome codes written to out_status. */
 #define NBL_OUT_ALIVE       0

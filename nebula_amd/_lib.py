@@ -29,6 +29,7 @@ MAT_DETECTOR_LT50, MAT_DETECTOR_GE50, MAT_VACUUM, MAT_MIRROR = -125, -124, -123,
 
 TAB_ELASTIC_IMFP, TAB_INELASTIC_IMFP = 0, 1
 TAB_ELASTIC_ICDF, TAB_INELASTIC_W0_ICDF, TAB_IONISATION = 2, 3, 4
+TAB_PENN_Q_ICDF, TAB_PENN_IONISATION = 5, 6
 EVENT_SAMPLE_PATH, EVENT_ELASTIC, EVENT_INELASTIC, EVENT_INTERSECT = 0, 1, 2, 3
 OUT_ALIVE, OUT_DETECTED, OUT_TERMINATED = 0, 1, 2
 NUM_UNIFORMS = 8
@@ -59,12 +60,37 @@ class Table2D(C.Structure):
                 ("data", C.POINTER(C.c_float))]
 
 
+class Table3D(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("depth", C.c_int32),
+                ("x_min", C.c_float), ("x_max", C.c_float), ("y_min", C.c_float), ("y_max", C.c_float),
+                ("z_min", C.c_float), ("z_max", C.c_float), ("data", C.POINTER(C.c_float))]
+
+
+INELASTIC_KIEFT, INELASTIC_PENN = 0, 1
+
+
 class MaterialDesc(C.Structure):
     _fields_ = [("barrier", C.c_float), ("fermi", C.c_float), ("band_gap", C.c_float),
                 ("phonon_loss", C.c_float), ("recoil_const", C.c_float),
                 ("elastic_log_imfp", Table1D), ("elastic_costheta_icdf", Table2D),
                 ("inelastic_log_imfp", Table1D), ("inelastic_log_w0_icdf", Table2D),
-                ("ionisation", Table2D)]
+                ("ionisation", Table2D), ("inelastic_model", C.c_int32),
+                ("penn_q_icdf", Table3D), ("penn_ionisation", Table3D)]
+
+
+class PennRaw(C.Structure):
+    _fields_ = [("fermi", C.c_float), ("band_gap", C.c_float),
+                ("n_k", C.c_int32), ("k_min", C.c_double), ("k_max", C.c_double),
+                ("n_p", C.c_int32), ("p_min", C.c_double), ("p_max", C.c_double),
+                ("imfp", C.POINTER(C.c_float)), ("omega_icdf", C.POINTER(C.c_float)),
+                ("q_nk", C.c_int32), ("q_nw", C.c_int32), ("q_np", C.c_int32),
+                ("q_k_min", C.c_double), ("q_k_max", C.c_double), ("q_w_min", C.c_double),
+                ("q_w_max", C.c_double), ("q_p_min", C.c_double), ("q_p_max", C.c_double),
+                ("q_icdf", C.POINTER(C.c_float)),
+                ("i_nk", C.c_int32), ("i_nw", C.c_int32), ("i_np", C.c_int32),
+                ("i_k_min", C.c_double), ("i_k_max", C.c_double), ("i_w_min", C.c_double),
+                ("i_w_max", C.c_double), ("i_p_min", C.c_double), ("i_p_max", C.c_double),
+                ("binding_icdf", C.POINTER(C.c_float))]
 
 
 class KieftRaw(C.Structure):
@@ -182,6 +208,8 @@ SIGNATURES = {
     "nbl_geometry_propagate_batch_device": [_P, _P, _P, _P, _P, _P, _U32, _P, _P],
     "nbl_kieft_prepare": [C.POINTER(KieftRaw), _P, _P, _P, _P, _P, C.POINTER(MaterialDesc)],
     "nbl_materials_create": [C.POINTER(MaterialDesc), _I32, C.POINTER(_P)],
+    "nbl_penn_prepare": [C.POINTER(PennRaw), _P, _P, _P, _P, C.POINTER(MaterialDesc)],
+    "nbl_table3d_get_batch": [_P, _I32, _I32, _I32, _P, _P, _P, _U32, _P],
     "nbl_materials_destroy": [_P],
     "nbl_materials_get_max_energy": [_P, C.POINTER(_F)],
     "nbl_table1d_get_batch": [_P, _I32, _I32, _P, _U32, _P],

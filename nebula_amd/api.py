@@ -82,7 +82,8 @@ def octree_linearize(triangles):
 
 class PreparedMaterial:
     """A material's tables after the reference's create() post-processing
-    (nbl_kieft_prepare), held in numpy arrays; .desc points at them."""
+    (nbl_kieft_prepare, then nbl_penn_prepare for a full Penn material),
+    held in numpy arrays; .desc points at them."""
 
     def __init__(self, spec: synth.MaterialSpec):
         raw = synth.raw_material(spec)
@@ -98,6 +99,17 @@ class PreparedMaterial:
         self.desc = L.MaterialDesc()
         call("nbl_kieft_prepare", C.byref(self.raw_struct), ptr(self.el_imfp), ptr(self.el_icdf),
              ptr(self.in_imfp), ptr(self.in_w0), ptr(self.ion), C.byref(self.desc))
+        if spec.inelastic == "penn":
+            n = synth.PENN_N
+            self.in_imfp = np.zeros(nk, np.float32)
+            self.in_w0 = np.zeros((nk, np_), np.float32)
+            self.penn_q = np.zeros((n, n, n), np.float32)
+            self.penn_ion = np.zeros((n, n, n), np.float32)
+            self.penn_struct = raw.penn_struct(L.PennRaw)
+            call("nbl_penn_prepare", C.byref(self.penn_struct), ptr(self.in_imfp), ptr(self.in_w0),
+                 ptr(self.penn_q), ptr(self.penn_ion), C.byref(self.desc))
+        elif spec.inelastic != "kieft":
+            raise ValueError(f"unknown inelastic model {spec.inelastic!r}")
 
 
 _PREP: dict = {}
@@ -141,6 +153,13 @@ class Materials:
         x = np.ascontiguousarray(x, np.float32); y = np.ascontiguousarray(y, np.float32)
         out = np.zeros(len(x), np.float32)
         call("nbl_table2d_get_batch", self.h, material, table, int(rounddown), ptr(x), ptr(y), len(x), ptr(out))
+        return out
+
+    def table3d(self, material, table, x, y, z, rounddown=False):
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        out = np.zeros(len(x), np.float32)
+        call("nbl_table3d_get_batch", self.h, material, table, int(rounddown), ptr(x), ptr(y), ptr(z), len(x),
+             ptr(out))
         return out
 
     def physics(self, event, particles, material, uniforms, last_tri=None, geometry=None):

@@ -11,6 +11,7 @@ uint32_t geometry_n_input(nbl_geometry g);
 bool geometry_has_record(nbl_geometry g, uint32_t id);
 const dmat* materials_device(nbl_materials m);
 int materials_count(nbl_materials m);
+const dmat& materials_view(nbl_materials m, int i);
 float materials_max_energy(nbl_materials m);
 
 // Device view of the slot arrays (gpu_particle_manager.h:79-91), passed by value.

@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_events_b(dslots s, const dmat* __restri
     const int mat = s.mat[idx];
     rng_xorwow r{ s.rng[idx] };
     dparticle q;
-    const bool made = inelastic_execute(mats[mat], p, r, &q);
+    const bool made = inelastic_event(mats[mat], p, r, &q);
     s.rng[idx] = r.s;
     s.parts[idx] = p;
     if (made) {   // create_secondary (gpu_particle_manager.inl:359-369)

@@ -49,11 +49,31 @@ dtab2 make_t2(const nbl_table2d_desc& t, const float* dev)
     return { dev, t.width, t.height, t.x_min, (float)(t.width - 1) / (t.x_max - t.x_min),
              t.y_min, (float)(t.height - 1) / (t.y_max - t.y_min) };
 }
+void check_t3(const nbl_table3d_desc& t, const char* what)
+{
+    if (t.width < 2 || t.height < 2 || t.depth < 2 || !t.data || !std::isfinite(t.x_min) || !std::isfinite(t.x_max) ||
+        !std::isfinite(t.y_min) || !std::isfinite(t.y_max) || !std::isfinite(t.z_min) || !std::isfinite(t.z_max) ||
+        t.x_max == t.x_min || t.y_max == t.y_min || t.z_max == t.z_min)
+        throw invalid_arg(std::string("nbl_materials_create: invalid 3D table ") + what);
+}
+// table_3D::create axis steps (table_3D.inl:16-22)
+dtab3 make_t3(const nbl_table3d_desc& t, const float* dev)
+{
+    return { dev, t.width, t.height, t.depth, t.x_min, (float)(t.width - 1) / (t.x_max - t.x_min),
+             t.y_min, (float)(t.height - 1) / (t.y_max - t.y_min), t.z_min, (float)(t.depth - 1) / (t.z_max - t.z_min) };
+}
+size_t t3_size(const nbl_table3d_desc& t) { return (size_t)t.width * t.height * t.depth; }
 
 __global__ __launch_bounds__(256) void k_tab1(dtab1 t, const float* __restrict__ x, uint32_t n, float* __restrict__ out)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         out[i] = tab1_get(t, x[i]);
+}
+__global__ __launch_bounds__(256) void k_tab3(dtab3 t, int rounddown, const float* __restrict__ x,
+    const float* __restrict__ y, const float* __restrict__ z, uint32_t n, float* __restrict__ out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = rounddown ? tab3_rounddown(t, x[i], y[i], z[i]) : tab3_get(t, x[i], y[i], z[i]);
 }
 __global__ __launch_bounds__(256) void k_tab2(dtab2 t, int rounddown, const float* __restrict__ x,
     const float* __restrict__ y, uint32_t n, float* __restrict__ out)
@@ -86,7 +106,7 @@ __global__ __launch_bounds__(256) void k_physics(phys_args a)
         } else if (a.event == NBL_EVENT_ELASTIC) {
             elastic_execute(a.mats[mat], p, rng);
         } else if (a.event == NBL_EVENT_INELASTIC) {
-            has = inelastic_execute(a.mats[mat], p, rng, &sec) ? 1 : 0;
+            has = inelastic_event(a.mats[mat], p, rng, &sec) ? 1 : 0;
         } else {
             st = (uint8_t)intersect_execute(a.mats, a.tris[a.last_tri[i]], p, mat, rng);
         }
@@ -174,6 +194,69 @@ int nbl_kieft_prepare(const nbl_kieft_raw* r, float* el_imfp, float* el_icdf, fl
         d->inelastic_log_imfp = { nk, lk_lo, lk_hi, in_imfp };
         d->inelastic_log_w0_icdf = { nk, np, lk_lo, lk_hi, p_lo, p_hi, in_w0 };
         d->ionisation = { nk, np, lk_lo, lk_hi, p_lo, p_hi, ion };
+        d->inelastic_model = NBL_INELASTIC_KIEFT;
+        d->penn_q_icdf = nbl_table3d_desc{};
+        d->penn_ionisation = nbl_table3d_desc{};
+    });
+}
+
+int nbl_penn_prepare(const nbl_penn_raw* r, float* log_imfp, float* log_omega_icdf, float* q_icdf, float* ionisation,
+    nbl_material_desc* d)
+{
+    return guarded([&] {
+        require(r && log_imfp && log_omega_icdf && q_icdf && ionisation && d, "nbl_penn_prepare: NULL argument");
+        require(r->imfp && r->omega_icdf && r->q_icdf && r->binding_icdf, "nbl_penn_prepare: NULL table");
+        require(r->n_k >= 2 && r->n_p >= 2 && r->q_nk >= 2 && r->q_nw >= 2 && r->q_np >= 2 && r->i_nk >= 2 &&
+                r->i_nw >= 2 && r->i_np >= 2, "nbl_penn_prepare: need at least 2 points per axis");
+        require(r->k_min > 0 && r->k_max > r->k_min && r->p_max > r->p_min && r->q_k_min > 0 &&
+                r->q_k_max > r->q_k_min && r->q_w_max > r->q_w_min && r->q_p_max > r->q_p_min && r->i_k_min > 0 &&
+                r->i_k_max > r->i_k_min && r->i_w_min > 0 && r->i_w_max > r->i_w_min && r->i_p_max > r->i_p_min,
+            "nbl_penn_prepare: bad axis range");
+        // full_penn.h:304-318: log of the IMFP in nm^-1
+        const int nk = r->n_k, np = r->n_p;
+        std::vector<double> K(nk);
+        const double ks = std::log(r->k_max / r->k_min) / (nk - 1);
+        for (int i = 0; i < nk; ++i) K[i] = r->k_min * std::exp(i * ks);
+        const float lk_lo = (float)std::log(K[0]), lk_hi = (float)std::log(K[nk - 1]);
+        const float p_lo = (float)r->p_min, p_hi = (float)(r->p_min + (np - 1) * ((r->p_max - r->p_min) / (np - 1)));
+        for (int i = 0; i < nk; ++i) log_imfp[i] = std::log(r->imfp[i] * 1.0f);
+        // full_penn.h:320-341: log(omega), omega clamped to [0, K - E_F]
+        const double fermi = (double)r->fermi;
+        for (int i = 0; i < nk; ++i)
+            for (int j = 0; j < np; ++j) {
+                const size_t k = (size_t)i * np + j;
+                log_omega_icdf[k] = (float)std::log(std::max(0.0, std::min<double>(K[i] - fermi, (double)(r->omega_icdf[k] * 1.0f))));
+            }
+        // full_penn.h:344-366: q in units of hbar / sqrt(2 m) (0.19519 nm eV^1/2)
+        const size_t nq = (size_t)r->q_nk * r->q_nw * r->q_np;
+        const float q_unit = (float)(0.19519 * 1.0);
+        for (size_t k = 0; k < nq; ++k) q_icdf[k] = r->q_icdf[k] * q_unit;
+        // full_penn.h:369-396: binding energies below 50 eV (or not finite) -> -1
+        const size_t ni = (size_t)r->i_nk * r->i_nw * r->i_np;
+        for (size_t k = 0; k < ni; ++k) {
+            float b = r->binding_icdf[k] * 1.0f;
+            if (b < 50 || !std::isfinite(b)) b = -1;
+            ionisation[k] = b;
+        }
+        auto lin_hi = [](double lo, double hi, int n) { return (float)(lo + (n - 1) * ((hi - lo) / (n - 1))); };
+        auto log_end = [](double lo, double hi, int n, int i) { return lo * std::exp(i * (std::log(hi / lo) / (n - 1))); };
+        d->fermi = r->fermi;
+        d->band_gap = r->band_gap;
+        d->inelastic_log_imfp = { nk, lk_lo, lk_hi, log_imfp };
+        d->inelastic_log_w0_icdf = { nk, np, lk_lo, lk_hi, p_lo, p_hi, log_omega_icdf };
+        d->ionisation = nbl_table2d_desc{};
+        d->penn_q_icdf = { r->q_nk, r->q_nw, r->q_np,
+            (float)std::log(log_end(r->q_k_min, r->q_k_max, r->q_nk, 0)),
+            (float)std::log(log_end(r->q_k_min, r->q_k_max, r->q_nk, r->q_nk - 1)),
+            (float)r->q_w_min, lin_hi(r->q_w_min, r->q_w_max, r->q_nw),
+            (float)r->q_p_min, lin_hi(r->q_p_min, r->q_p_max, r->q_np), q_icdf };
+        d->penn_ionisation = { r->i_nk, r->i_nw, r->i_np,
+            (float)std::log(log_end(r->i_k_min, r->i_k_max, r->i_nk, 0)),
+            (float)std::log(log_end(r->i_k_min, r->i_k_max, r->i_nk, r->i_nk - 1)),
+            (float)std::log(log_end(r->i_w_min, r->i_w_max, r->i_nw, 0)),
+            (float)std::log(log_end(r->i_w_min, r->i_w_max, r->i_nw, r->i_nw - 1)),
+            (float)r->i_p_min, lin_hi(r->i_p_min, r->i_p_max, r->i_np), ionisation };
+        d->inelastic_model = NBL_INELASTIC_PENN;
     });
 }
 
@@ -193,22 +276,32 @@ int nbl_materials_create(const nbl_material_desc* mats, int32_t n, nbl_materials
             check_t1(d.inelastic_log_imfp, "inelastic_log_imfp");
             check_t2(d.elastic_costheta_icdf, "elastic_costheta_icdf");
             check_t2(d.inelastic_log_w0_icdf, "inelastic_log_w0_icdf");
-            check_t2(d.ionisation, "ionisation");
+            require(d.inelastic_model == NBL_INELASTIC_KIEFT || d.inelastic_model == NBL_INELASTIC_PENN,
+                "nbl_materials_create: unknown inelastic model");
+            require(d.inelastic_model == mats[0].inelastic_model,
+                "nbl_materials_create: all materials of a set must use the same inelastic model");
             total += d.elastic_log_imfp.n + d.inelastic_log_imfp.n +
                      (size_t)d.elastic_costheta_icdf.width * d.elastic_costheta_icdf.height +
-                     (size_t)d.inelastic_log_w0_icdf.width * d.inelastic_log_w0_icdf.height +
-                     (size_t)d.ionisation.width * d.ionisation.height;
+                     (size_t)d.inelastic_log_w0_icdf.width * d.inelastic_log_w0_icdf.height;
+            if (d.inelastic_model == NBL_INELASTIC_KIEFT) {
+                check_t2(d.ionisation, "ionisation");
+                total += (size_t)d.ionisation.width * d.ionisation.height;
+            } else {
+                check_t3(d.penn_q_icdf, "penn_q_icdf");
+                check_t3(d.penn_ionisation, "penn_ionisation");
+                total += t3_size(d.penn_q_icdf) + t3_size(d.penn_ionisation);
+            }
         }
         // one allocation; every table starts on a 256-byte boundary
         std::vector<float> host;
-        host.reserve(total + 64 * 5 * n);
+        host.reserve(total + 64 * 7 * n);
         auto put = [&](const float* src, size_t cnt) {
             while (host.size() % 64) host.push_back(0.f);
             const size_t off = host.size();
             host.insert(host.end(), src, src + cnt);
             return off;
         };
-        struct offs { size_t a, b, c, d, e; };
+        struct offs { size_t a, b, c, d, e, f, g; };
         std::vector<offs> o(n);
         for (int i = 0; i < n; ++i) {
             const nbl_material_desc& d = mats[i];
@@ -216,7 +309,13 @@ int nbl_materials_create(const nbl_material_desc* mats, int32_t n, nbl_materials
             o[i].b = put(d.inelastic_log_imfp.data, d.inelastic_log_imfp.n);
             o[i].c = put(d.elastic_costheta_icdf.data, (size_t)d.elastic_costheta_icdf.width * d.elastic_costheta_icdf.height);
             o[i].d = put(d.inelastic_log_w0_icdf.data, (size_t)d.inelastic_log_w0_icdf.width * d.inelastic_log_w0_icdf.height);
-            o[i].e = put(d.ionisation.data, (size_t)d.ionisation.width * d.ionisation.height);
+            o[i].e = o[i].f = o[i].g = 0;
+            if (d.inelastic_model == NBL_INELASTIC_KIEFT) {
+                o[i].e = put(d.ionisation.data, (size_t)d.ionisation.width * d.ionisation.height);
+            } else {
+                o[i].f = put(d.penn_q_icdf.data, t3_size(d.penn_q_icdf));
+                o[i].g = put(d.penn_ionisation.data, t3_size(d.penn_ionisation));
+            }
         }
         m->data.alloc(host.size());
         m->data.upload(host.data(), host.size());
@@ -230,7 +329,16 @@ int nbl_materials_create(const nbl_material_desc* mats, int32_t n, nbl_materials
             v.in_imfp = make_t1(d.inelastic_log_imfp, m->data.p + o[i].b);
             v.el_icdf = make_t2(d.elastic_costheta_icdf, m->data.p + o[i].c);
             v.in_icdf = make_t2(d.inelastic_log_w0_icdf, m->data.p + o[i].d);
-            v.ion = make_t2(d.ionisation, m->data.p + o[i].e);
+            v.model = d.inelastic_model;
+            v.ion = dtab2{};
+            v.pq = dtab3{};
+            v.pion = dtab3{};
+            if (v.model == INEL_KIEFT) {
+                v.ion = make_t2(d.ionisation, m->data.p + o[i].e);
+            } else {
+                v.pq = make_t3(d.penn_q_icdf, m->data.p + o[i].f);
+                v.pion = make_t3(d.penn_ionisation, m->data.p + o[i].g);
+            }
             m->views.push_back(v);
             // get_max_energy: exp(scalemax) of both IMFP tables, minimum over
             // mechanisms and materials (elastic.h:46-49, cpu_material_manager.inl:42-55)
@@ -290,10 +398,33 @@ int nbl_table2d_get_batch(nbl_materials m, int32_t material, int32_t table, int3
         require(n == 0 || (x && y && out), "nbl_table2d_get_batch: NULL array");
         if (n == 0) return;
         const dmat& v = m->views[material];
+        require(table != NBL_TAB_IONISATION || v.model == INEL_KIEFT,
+            "nbl_table2d_get_batch: a full Penn material has no 2D ionisation table");
         const dtab2& t = table == NBL_TAB_ELASTIC_ICDF ? v.el_icdf : table == NBL_TAB_INELASTIC_W0_ICDF ? v.in_icdf : v.ion;
         dbuf<float> dx(n), dy(n), dout(n);
         dx.upload(x, n); dy.upload(y, n);
         hipLaunchKernelGGL(k_tab2, dim3(grid_for(n, 256)), dim3(256), 0, 0, t, rounddown ? 1 : 0, dx.p, dy.p, n, dout.p);
+        NBL_HIP(hipGetLastError());
+        dout.download(out, n);
+        NBL_HIP(hipDeviceSynchronize());
+    });
+}
+
+int nbl_table3d_get_batch(nbl_materials m, int32_t material, int32_t table, int32_t rounddown,
+    const float* x, const float* y, const float* z, uint32_t n, float* out)
+{
+    return guarded([&] {
+        require(m != nullptr, "nbl_table3d_get_batch: NULL materials");
+        require(material >= 0 && material < (int)m->views.size(), "nbl_table3d_get_batch: bad material");
+        require(table == NBL_TAB_PENN_Q_ICDF || table == NBL_TAB_PENN_IONISATION, "nbl_table3d_get_batch: bad table");
+        const dmat& v = m->views[material];
+        require(v.model == INEL_PENN, "nbl_table3d_get_batch: not a full Penn material");
+        require(n == 0 || (x && y && z && out), "nbl_table3d_get_batch: NULL array");
+        if (n == 0) return;
+        const dtab3& t = table == NBL_TAB_PENN_Q_ICDF ? v.pq : v.pion;
+        dbuf<float> dx(n), dy(n), dz(n), dout(n);
+        dx.upload(x, n); dy.upload(y, n); dz.upload(z, n);
+        hipLaunchKernelGGL(k_tab3, dim3(grid_for(n, 256)), dim3(256), 0, 0, t, rounddown ? 1 : 0, dx.p, dy.p, dz.p, n, dout.p);
         NBL_HIP(hipGetLastError());
         dout.download(out, n);
         NBL_HIP(hipDeviceSynchronize());

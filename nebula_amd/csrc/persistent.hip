@@ -148,8 +148,10 @@ __device__ void make_primary(const pers_params& P, uint64_t i, dparticle& p, uin
 // The learned variant needs ~150 registers while the networks run, so it is
 // built for 12 wavefronts per CU (3 per SIMD) instead of 16.
 // DEPOSIT: the energy-deposition tally is on (a separate instantiation, so
-// that the plain kernel carries none of its registers).
-template<bool LEARNED, bool DEPOSIT>
+// that the plain kernel carries none of its registers).  PENN: the material
+// set's inelastic model is full Penn (its event runs on its own, not merged
+// with the elastic lanes' direction frame).
+template<bool LEARNED, bool DEPOSIT, bool PENN>
 __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params P)
 {
     extern __shared__ float lds[];
@@ -308,7 +310,7 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
                     last = NO_TRI;
                     rot = scatter_begin_v(P.mats[mat], p, rng, el, v, ss);
                 }
-            } else if (el || go) {
+            } else if (el || (go && !PENN)) {
                 pend = false;
                 last = NO_TRI;
                 rot = scatter_begin(P.mats[mat], p, rng, el, ss);
@@ -316,6 +318,11 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
             f3 nd{ 0, 0, 0 };
             if (rot) nd = scatter_frame(p, rng);
             if (rot) made = scatter_end(P.mats[mat], p, rng, ss, nd, &q);
+            if (PENN && go) {
+                pend = false;
+                last = NO_TRI;
+                made = penn_execute(P.mats[mat], p, rng, &q);
+            }
             // energy deposit of a scatter event (energydeposit_cpu_driver.h:72-94)
             if (DEPOSIT) {
                 const float de = (el || go) ? e_before - p.e : 0.0f;
@@ -424,6 +431,8 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         require(ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
         require(block % 64 == 0 && block >= 64 && block <= 1024, "nbl_driver_run_persistent: block_size must be 64..1024, multiple of 64");
         require(!d->sampler || block <= 768, "nbl_driver_run_persistent: block_size must be <= 768 with a learned sampler");
+        const bool penn = materials_view(d->mats, 0).model == INEL_PENN;
+        require(!(penn && d->sampler), "nbl_driver_run_persistent: the learned sampler covers Kieft inelastic tables only");
         require(per_cu >= 1 && per_cu <= 32, "nbl_driver_run_persistent: blocks_per_cu must be 1..32");
         require(depth >= 64 && depth <= 1024, "nbl_driver_run_persistent: stack_depth must be 64..1024");
         require(thresh >= 1 && thresh <= 64, "nbl_driver_run_persistent: refill_threshold must be 1..64");
@@ -488,11 +497,14 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         if (d->sampler) {
             const dmlp* v = sampler_views(d->sampler);
             P.mlp[0] = v[0]; P.mlp[1] = v[1];
-            if (dg) hipLaunchKernelGGL((k_persistent<true, true>), grid, threads_per_block, lds, s, P);
-            else hipLaunchKernelGGL((k_persistent<true, false>), grid, threads_per_block, lds, s, P);
+            if (dg) hipLaunchKernelGGL((k_persistent<true, true, false>), grid, threads_per_block, lds, s, P);
+            else hipLaunchKernelGGL((k_persistent<true, false, false>), grid, threads_per_block, lds, s, P);
+        } else if (penn) {
+            if (dg) hipLaunchKernelGGL((k_persistent<false, true, true>), grid, threads_per_block, lds, s, P);
+            else hipLaunchKernelGGL((k_persistent<false, false, true>), grid, threads_per_block, lds, s, P);
         } else {
-            if (dg) hipLaunchKernelGGL((k_persistent<false, true>), grid, threads_per_block, lds, s, P);
-            else hipLaunchKernelGGL((k_persistent<false, false>), grid, threads_per_block, lds, s, P);
+            if (dg) hipLaunchKernelGGL((k_persistent<false, true, false>), grid, threads_per_block, lds, s, P);
+            else hipLaunchKernelGGL((k_persistent<false, false, false>), grid, threads_per_block, lds, s, P);
         }
         NBL_HIP(hipGetLastError());
         NBL_HIP(hipEventRecord(e1, s));

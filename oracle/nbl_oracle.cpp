@@ -580,10 +580,48 @@ struct table2d {
     }
 };
 
+/* table_3D (common/util/table_3D.inl): value(i,j,k) at (i*height + j)*depth + k. */
+struct table3d {
+    int w = 0, h = 0, dp = 0;
+    float x_min = 0, x_step = 0, y_min = 0, y_step = 0, z_min = 0, z_step = 0;
+    std::vector<float> d;
+    void create(float xmin, float xmax, int w_, float ymin, float ymax, int h_, float zmin, float zmax, int dp_,
+        const float* data)                              /* table_3D.inl:5-25 */
+    {
+        w = w_; h = h_; dp = dp_; d.assign(data, data + (size_t)w * h * dp);
+        x_min = xmin; x_step = (w - 1) / (xmax - xmin);
+        y_min = ymin; y_step = (h - 1) / (ymax - ymin);
+        z_min = zmin; z_step = (dp - 1) / (zmax - zmin);
+    }
+    float at(int i, int j, int k) const { return d[((size_t)i * h + j) * dp + k]; }   /* table_3D.inl:64-74 */
+    float get(float x, float y, float z) const          /* table_3D.inl:76-106 */
+    {
+        const float xi = (x - x_min) * x_step, yi = (y - y_min) * y_step, zi = (z - z_min) * z_step;
+        const int lx = clampi(floorr(xi), 0, w - 2), ly = clampi(floorr(yi), 0, h - 2), lz = clampi(floorr(zi), 0, dp - 2);
+        const float fx = xi - lx, fy = yi - ly, fz = zi - lz;
+        return (1 - fx) * (1 - fy) * (1 - fz) * at(lx, ly, lz)
+            + fx * (1 - fy) * (1 - fz) * at(lx + 1, ly, lz)
+            + (1 - fx) * fy * (1 - fz) * at(lx, ly + 1, lz)
+            + fx * fy * (1 - fz) * at(lx + 1, ly + 1, lz)
+            + (1 - fx) * (1 - fy) * fz * at(lx, ly, lz + 1)
+            + fx * (1 - fy) * fz * at(lx + 1, ly, lz + 1)
+            + (1 - fx) * fy * fz * at(lx, ly + 1, lz + 1)
+            + fx * fy * fz * at(lx + 1, ly + 1, lz + 1);
+    }
+    float rounddown(float x, float y, float z) const    /* table_3D.inl:108-118 */
+    {
+        const float xi = (x - x_min) * x_step, yi = (y - y_min) * y_step, zi = (z - z_min) * z_step;
+        return at(clampi(floorr(xi), 0, w - 1), clampi(floorr(yi), 0, h - 1), clampi(floorr(zi), 0, dp - 1));
+    }
+};
+
 /* ---- material (core/material.h, physics/kieft/elastic.h, inelastic.h) ----- */
+/* For full Penn (physics/full_penn.h) in_imfp / in_icdf hold the log IMFP and
+ * log omega ICDF, pq the q ICDF and pion the 3D binding ICDF; ion is unused. */
 struct material {
     float barrier = 0, fermi = 0, band_gap = -1, phonon_loss = 0, recoil_const = 0;
     table1d el_imfp; table2d el_icdf; table1d in_imfp; table2d in_icdf; table2d ion;
+    int model = NBL_INELASTIC_KIEFT; table3d pq, pion;
     /* kieft_*::get_max_energy and the energy_visitor min (scatter_list.h:135-146):
      * for_each visits elastic (index 1) first, then inelastic. */
     float max_energy() const
@@ -725,6 +763,137 @@ static inline bool inelastic_execute(const material& m, particle& p, R& rng, par
     return true;
 }
 
+/* The reference's plasmon-decay rejection loop (full_penn.h:173-184) runs
+ * until acceptance.  Both sides of this project stop after this many
+ * proposals and keep the last one; with random uniforms the acceptance
+ * probability per proposal exceeds 0.6, so the cap only bounds the loop for
+ * fixed injected uniforms. */
+static constexpr int PENN_MAX_PROPOSALS = 1000;
+
+/* Plasmon decay's x = E_i / omega (full_penn.h:141-187): p(x) ~ sqrt(x (x + 1))
+ * on [max(0, f - 1), f], f = E_F / omega; for f < 0.1, p(x) ~ sqrt(x) on [0, f]. */
+template<typename R>
+static inline float plasmon_x(float f, R& rng)
+{
+    if (f < 0.1f) {
+        const float U = std::cbrt(rng.unit());
+        return f * U * U;
+    }
+    const float lower = maxr(0, f - 1);
+    const float upper = f;
+    const float d1 = upper - lower;
+    const float d2 = upper * upper - lower * lower;
+    const float alpha = d2 / (d1 + d2);
+    float x;
+    int tries = 0;
+    for (;;) {
+        const float U1 = rng.unit();
+        const float U2 = rng.unit();
+        x = (U1 < alpha) ? std::sqrt(d2 * U2 + lower * lower) : d1 * U2 + lower;
+        if (!(rng.unit() > std::sqrt(x * x + x) / (x + 0.5f))) break;
+        if (++tries >= PENN_MAX_PROPOSALS) break;
+    }
+    return x;
+}
+
+/* full_penn<false, true>::execute (physics/full_penn.h:61-288).
+ * Returns true and fills sec if a secondary electron is created. */
+template<typename R>
+static inline bool penn_execute(const material& m, particle& p, R& rng, particle& sec, counters* c)
+{
+    float omega;
+    {   /* full_penn.h:73-79 */
+        const float x = std::log(p.kin_energy);
+        const float y = rng.unit();
+        omega = std::exp(m.in_icdf.get(x, y));
+    }
+    float binding;
+    {   /* full_penn.h:81-94: rounddown at (log K, log(omega/K), P) */
+        const float x = std::log(p.kin_energy);
+        const float y = std::log(omega / p.kin_energy);
+        const float z = rng.unit();
+        binding = m.pion.rounddown(x, y, z);
+    }
+    if (c) { c->lookups_2d += 1; c->rounddowns += 1; }
+    normalise(p.dir);
+    if (binding <= 0) {
+        if (!(omega > m.band_gap)) {      /* full_penn.h:226-234: sub-band-gap loss */
+            p.kin_energy -= omega;
+            return false;
+        }
+        float q;                           /* full_penn.h:104-113 */
+        {
+            const float x = std::log(p.kin_energy);
+            const float y = omega / p.kin_energy;
+            const float z = rng.unit();
+            q = m.pq.get(x, y, z);
+        }
+        const vec3 prim_normal_dir = normalised(make_normal_vec(p.dir, rng.phi()));
+        vec3 q_dir, q_normal_dir;
+        {   /* full_penn.h:115-123 */
+            const float ct = clampr((omega + q * q) / (2 * std::sqrt(p.kin_energy) * q), -1, 1);
+            const float st = std::sqrt(1 - ct * ct);
+            q_dir = normalised(p.dir * ct + prim_normal_dir * st);
+            q_normal_dir = normalised(make_normal_vec(q_dir, rng.phi()));
+        }
+        {   /* full_penn.h:126-131: deflect the primary */
+            const float ct = clampr((2 * p.kin_energy - omega - q * q) /
+                (2 * std::sqrt(p.kin_energy * (p.kin_energy - omega))), -1, 1);
+            const float st = std::sqrt(1 - ct * ct);
+            p.kin_energy -= omega;
+            p.dir = p.dir * ct + prim_normal_dir * st;
+        }
+        const float k_fermi = std::sqrt(m.fermi);
+        const float k_fermi_omega = std::sqrt(m.fermi + omega);
+        if (q < k_fermi_omega - k_fermi) {
+            /* plasmon decay, full_penn.h:141-187 */
+            const float x = plasmon_x(m.fermi / omega, rng);
+            sec.kin_energy = x * omega + omega;
+            sec.dir = rng.uniform_vector();
+        } else if (q < k_fermi_omega + k_fermi) {
+            /* single excitation, full_penn.h:188-214 */
+            const float ki_parr = (omega - q * q) / (2 * q);
+            const float kf_parr = ki_parr + q;
+            const float ki_perp_min_sq = maxr(0, m.fermi - kf_parr * kf_parr);
+            const float ki_perp_max_sq = m.fermi - ki_parr * ki_parr;
+            const float ki_perp_sq = maxr(0, ki_perp_min_sq + (ki_perp_max_sq - ki_perp_min_sq) * rng.unit());
+            const vec3 ki = ki_parr * q_dir + std::sqrt(ki_perp_sq) * q_normal_dir;
+            sec.kin_energy = dot(ki, ki) + omega;
+            sec.dir = ki + q * q_dir;
+        } else {
+            return false;                  /* full_penn.h:215-221: no secondary */
+        }
+        sec.pos = p.pos;
+        if (c) c->secondaries += 1;
+        return true;
+    }
+    /* inner-shell excitation, full_penn.h:237-285 */
+    const vec3 normal_dir = normalised(make_normal_vec(p.dir, rng.phi()));
+    const float _K = p.kin_energy - m.fermi + 2 * binding;
+    const float dK = binding + omega;
+    const float cos_theta = std::sqrt(dK / _K);
+    const float sin_theta = std::sqrt(1 - saturater(cos_theta * cos_theta));
+    vec3 secondary_dir = p.dir * cos_theta + normal_dir * sin_theta;
+    normalise(secondary_dir);
+    secondary_dir += std::sqrt(binding / dK) * rng.uniform_vector();
+    normalise(secondary_dir);
+    sec.kin_energy = m.fermi + omega - binding;
+    sec.pos = p.pos;
+    sec.dir = secondary_dir;
+    p.kin_energy -= omega;
+    p.dir -= cos_theta * secondary_dir;
+    if (c) c->secondaries += 1;
+    return true;
+}
+
+/* The material's inelastic mechanism (physics_config.h:21-24 selects it at
+ * compile time in the reference; here it is a per-set property). */
+template<typename R>
+static inline bool inelastic_event(const material& m, particle& p, R& rng, particle& sec, counters* c)
+{
+    return m.model == NBL_INELASTIC_PENN ? penn_execute(m, p, rng, sec, c) : inelastic_execute(m, p, rng, sec, c);
+}
+
 /* boundary_intersect<true,true,false>::execute (boundary_intersect.h:39-156).
  * Returns NBL_OUT_*; *mat is the particle's material (updated on transmission). */
 template<typename R>
@@ -823,6 +992,64 @@ static int kieft_prepare(const nbl_kieft_raw& r, float* el_imfp, float* el_icdf,
     d.inelastic_log_imfp = { nk, lk0, lk1, in_imfp };
     d.inelastic_log_w0_icdf = { nk, np, lk0, lk1, p0, p1, in_w0 };
     d.ionisation = { nk, np, lk0, lk1, p0, p1, ion };
+    return 0;
+}
+
+/* ---- full Penn table preparation (physics/full_penn.h:296-399) ------------ */
+/* geomspace(lo, hi, n)[i] and linspace(lo, hi, n)[i] (common/util/range.h:204-312) */
+static inline double geom_at(double lo, double hi, int n, int i) { return lo * std::exp(i * (std::log(hi / lo) / (n - 1))); }
+static inline double lin_at(double lo, double hi, int n, int i) { return lo + i * ((hi - lo) / (n - 1)); }
+
+static int penn_prepare(const nbl_penn_raw& r, float* log_imfp, float* log_omega, float* q_icdf, float* ion,
+    nbl_material_desc& d)
+{
+    if (r.n_k < 2 || r.n_p < 2 || r.q_nk < 2 || r.q_nw < 2 || r.q_np < 2 || r.i_nk < 2 || r.i_nw < 2 || r.i_np < 2)
+        return -1;
+    if (!(r.k_min > 0) || !(r.k_max > r.k_min) || !(r.p_max > r.p_min) || !(r.q_k_min > 0) ||
+        !(r.q_k_max > r.q_k_min) || !(r.q_w_max > r.q_w_min) || !(r.q_p_max > r.q_p_min) || !(r.i_k_min > 0) ||
+        !(r.i_k_max > r.i_k_min) || !(r.i_w_min > 0) || !(r.i_w_max > r.i_w_min) || !(r.i_p_max > r.i_p_min))
+        return -1;
+    const int nk = r.n_k, np = r.n_p;
+    /* full_penn.h:304-318: log(imfp * unit), unit = 1 nm^-1, in float */
+    for (int i = 0; i < nk; ++i) log_imfp[i] = std::log(r.imfp[i] * 1.0f);
+    /* full_penn.h:320-341: log(max(0, min(K - fermi, omega))) in double */
+    for (int i = 0; i < nk; ++i) {
+        const double K = geom_at(r.k_min, r.k_max, nk, i);
+        for (int j = 0; j < np; ++j)
+            log_omega[(size_t)i * np + j] = (float)std::log(std::max(0.0,
+                std::min<double>(K - (double)r.fermi, (double)(r.omega_icdf[(size_t)i * np + j] * 1.0f))));
+    }
+    /* full_penn.h:344-366: q * (hbar / sqrt(2 m) = 0.19519 nm eV^1/2), the
+     * product taken in float with the unit rounded to float */
+    const float q_unit = (float)(1.0 * 0.19519 * 1.0);
+    for (size_t k = 0; k < (size_t)r.q_nk * r.q_nw * r.q_np; ++k) q_icdf[k] = r.q_icdf[k] * q_unit;
+    /* full_penn.h:369-396: bindings below 50 eV or not finite become -1 */
+    for (size_t k = 0; k < (size_t)r.i_nk * r.i_nw * r.i_np; ++k) {
+        const float b = r.binding_icdf[k] * 1.0f;
+        ion[k] = (b < 50 || !std::isfinite(b)) ? -1.0f : b;
+    }
+    d.fermi = r.fermi;
+    d.band_gap = r.band_gap;
+    const float lk0 = (float)std::log(geom_at(r.k_min, r.k_max, nk, 0));
+    const float lk1 = (float)std::log(geom_at(r.k_min, r.k_max, nk, nk - 1));
+    d.inelastic_log_imfp = { nk, lk0, lk1, log_imfp };
+    d.inelastic_log_w0_icdf = { nk, np, lk0, lk1, (float)lin_at(r.p_min, r.p_max, np, 0),
+                                (float)lin_at(r.p_min, r.p_max, np, np - 1), log_omega };
+    d.ionisation = nbl_table2d_desc{};
+    d.penn_q_icdf = { r.q_nk, r.q_nw, r.q_np,
+        (float)std::log(geom_at(r.q_k_min, r.q_k_max, r.q_nk, 0)),
+        (float)std::log(geom_at(r.q_k_min, r.q_k_max, r.q_nk, r.q_nk - 1)),
+        (float)lin_at(r.q_w_min, r.q_w_max, r.q_nw, 0), (float)lin_at(r.q_w_min, r.q_w_max, r.q_nw, r.q_nw - 1),
+        (float)lin_at(r.q_p_min, r.q_p_max, r.q_np, 0), (float)lin_at(r.q_p_min, r.q_p_max, r.q_np, r.q_np - 1),
+        q_icdf };
+    d.penn_ionisation = { r.i_nk, r.i_nw, r.i_np,
+        (float)std::log(geom_at(r.i_k_min, r.i_k_max, r.i_nk, 0)),
+        (float)std::log(geom_at(r.i_k_min, r.i_k_max, r.i_nk, r.i_nk - 1)),
+        (float)std::log(geom_at(r.i_w_min, r.i_w_max, r.i_nw, 0)),
+        (float)std::log(geom_at(r.i_w_min, r.i_w_max, r.i_nw, r.i_nw - 1)),
+        (float)lin_at(r.i_p_min, r.i_p_max, r.i_np, 0), (float)lin_at(r.i_p_min, r.i_p_max, r.i_np, r.i_np - 1),
+        ion };
+    d.inelastic_model = NBL_INELASTIC_PENN;
     return 0;
 }
 
@@ -961,7 +1188,7 @@ struct cpu_driver {
             particles[idx].data = p;
         } else {
             particle sec;
-            const bool made = inelastic_execute(m, p, rng, sec, &c);
+            const bool made = inelastic_event(m, p, rng, sec, &c);
             if (made) {   /* create_secondary (cpu_particle_manager.inl:181-196) */
                 const uint32_t tag = particles[idx].primary_tag;
                 const int mat = particles[idx].current_material;
@@ -1088,9 +1315,34 @@ orc_materials orc_materials_create(const nbl_material_desc* mats, int32_t n)
         table2d_from(t.el_icdf, mats[i].elastic_costheta_icdf);
         table1d_from(t.in_imfp, mats[i].inelastic_log_imfp);
         table2d_from(t.in_icdf, mats[i].inelastic_log_w0_icdf);
-        table2d_from(t.ion, mats[i].ionisation);
+        t.model = mats[i].inelastic_model;
+        if (t.model == NBL_INELASTIC_PENN) {
+            const nbl_table3d_desc& q = mats[i].penn_q_icdf;
+            const nbl_table3d_desc& b = mats[i].penn_ionisation;
+            t.pq.create(q.x_min, q.x_max, q.width, q.y_min, q.y_max, q.height, q.z_min, q.z_max, q.depth, q.data);
+            t.pion.create(b.x_min, b.x_max, b.width, b.y_min, b.y_max, b.height, b.z_min, b.z_max, b.depth, b.data);
+        } else {
+            table2d_from(t.ion, mats[i].ionisation);
+        }
     }
     return m;
+}
+void orc_penn_plasmon_x(float f, uint32_t n, uint64_t seed, float* out)
+{
+    xorwow st; xorwow_seed(st, seed, 0);
+    rng_xorwow rng(st);
+    for (uint32_t i = 0; i < n; ++i) out[i] = plasmon_x(f, rng);
+}
+int orc_penn_prepare(const nbl_penn_raw* raw, float* a, float* b, float* c, float* d, nbl_material_desc* desc)
+{
+    return penn_prepare(*raw, a, b, c, d, *desc);
+}
+void orc_table3d_get_batch(orc_materials m, int32_t mi, int32_t table, int32_t rd, const float* x,
+    const float* y, const float* z, uint32_t n, float* out)
+{
+    const material& t = m->m[mi];
+    const table3d& tab = table == NBL_TAB_PENN_Q_ICDF ? t.pq : t.pion;
+    for (uint32_t i = 0; i < n; ++i) out[i] = rd ? tab.rounddown(x[i], y[i], z[i]) : tab.get(x[i], y[i], z[i]);
 }
 void orc_materials_destroy(orc_materials m) { delete m; }
 float orc_materials_max_energy(orc_materials m)  /* cpu_material_manager.inl:42-55 */
@@ -1132,7 +1384,7 @@ int orc_physics_batch(orc_materials m, orc_geometry g, int32_t event, const int3
             elastic_execute(m->m[mat], p, rng, nullptr);
         } else if (event == NBL_EVENT_INELASTIC) {
             particle sec;
-            const bool made = inelastic_execute(m->m[mat], p, rng, sec, nullptr);
+            const bool made = inelastic_event(m->m[mat], p, rng, sec, nullptr);
             if (made && out_secondary) out_secondary[i] = to_nbl(sec);
             if (out_has_secondary) out_has_secondary[i] = made ? 1 : 0;
         } else if (event == NBL_EVENT_INTERSECT) {
@@ -1292,7 +1544,7 @@ void orc_slots_iteration(orc_slots o, orc_materials mm, orc_geometry gg, float m
         s.last_tri[idx] = NBL_NO_TRIANGLE;
         rng_xorwow r(s.rng[idx]);
         particle sec;
-        const bool made = inelastic_execute(mats[s.material[idx]], s.parts[idx], r, sec, nullptr);
+        const bool made = inelastic_event(mats[s.material[idx]], s.parts[idx], r, sec, nullptr);
         s.rng[idx] = r.st;
         if (made) {
             s.status[sec_idx] = NBL_ST_NEW_SECONDARY;

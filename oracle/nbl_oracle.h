@@ -51,6 +51,14 @@ void orc_table1d_get_batch(orc_materials m, int32_t material, int32_t table,
     const float* x, uint32_t n, float* out);
 void orc_table2d_get_batch(orc_materials m, int32_t material, int32_t table,
     int32_t rounddown, const float* x, const float* y, uint32_t n, float* out);
+/* full_penn::create's table prep (physics/full_penn.h:296-399). */
+int orc_penn_prepare(const nbl_penn_raw* raw, float* log_imfp, float* log_omega_icdf,
+    float* q_icdf, float* ionisation, nbl_material_desc* desc);
+/* n draws of full Penn's plasmon-decay x = E_i / omega at f = E_F / omega
+ * (full_penn.h:141-187) from one XORWOW stream. */
+void orc_penn_plasmon_x(float f, uint32_t n, uint64_t seed, float* out);
+void orc_table3d_get_batch(orc_materials m, int32_t material, int32_t table,
+    int32_t rounddown, const float* x, const float* y, const float* z, uint32_t n, float* out);
 /* Same contract as nbl_physics_batch. */
 int orc_physics_batch(orc_materials m, orc_geometry g, int32_t event,
     const int32_t* material, const uint32_t* last_tri, const float* uniforms,

@@ -8,7 +8,7 @@
  *   geometry/octree/octree_builder.cpp + tribox.cc (leaf sets, Morton order),
  *   geometry/octree.inl (in_domain, propagate),
  *   geometry/trilist.inl (brute-force propagate),
- *   common/util/table_1D.inl, table_2D.inl (get, get_rounddown),
+ *   common/util/table_1D.inl, table_2D.inl, table_3D.inl (get, get_rounddown),
  *   io/load_tri_file.cpp, io/load_pri_file.cpp (file formats, primary order)
  * bit for bit.  The physics headers are not included: they pull in the HDF5
  * material loader (io/hdf5_file.h), an external library.
@@ -31,6 +31,7 @@
 #include "geometry/trilist.h"
 #include "common/util/table_1D.h"
 #include "common/util/table_2D.h"
+#include "common/util/table_3D.h"
 #include "io/load_tri_file.h"
 #include "io/load_pri_file.h"
 
@@ -232,6 +233,17 @@ void ref_table2d_get_batch(int32_t w, int32_t h, float x_min, float x_max, float
     auto t = nbl::util::table_2D<float, false>::create(x_min, x_max, w, y_min, y_max, h, d.data());
     for (uint32_t i = 0; i < n; ++i) out[i] = rounddown ? t.get_rounddown(x[i], y[i]) : t.get(x[i], y[i]);
     nbl::util::table_2D<float, false>::destroy(t);
+}
+
+void ref_table3d_get_batch(int32_t w, int32_t h, int32_t dp, float x_min, float x_max, float y_min, float y_max,
+    float z_min, float z_max, const float* data, int32_t rounddown, const float* x, const float* y, const float* z,
+    uint32_t n, float* out)
+{
+    std::vector<float> d(data, data + (size_t)w * h * dp);
+    auto t = nbl::util::table_3D<float, false>::create(x_min, x_max, w, y_min, y_max, h, z_min, z_max, dp, d.data());
+    for (uint32_t i = 0; i < n; ++i)
+        out[i] = rounddown ? t.get_rounddown(x[i], y[i], z[i]) : t.get(x[i], y[i], z[i]);
+    nbl::util::table_3D<float, false>::destroy(t);
 }
 
 /* ---- io/load_tri_file.cpp, io/load_pri_file.cpp ---- */

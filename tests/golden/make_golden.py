@@ -110,6 +110,32 @@ def table_fixture():
     print("tables + intersect_ray fixture written")
 
 
+T3_SCALE = (-1.0, 4.0, 0.0, 1.0, -0.5, 0.5)      # x_min, x_max, y_min, y_max, z_min, z_max
+
+
+def table3d_fixture():
+    """table_3D get / get_rounddown on a random 13 x 17 x 11 table, queries
+    inside, on and outside every face (clamped extrapolation)."""
+    rng = np.random.default_rng(21)
+    w, h, dp = 13, 17, 11
+    d3 = rng.normal(0, 1, (w, h, dp)).astype(np.float32)
+    n = 6000
+    x = rng.uniform(-1.5, 4.5, n).astype(np.float32)
+    y = rng.uniform(-0.2, 1.2, n).astype(np.float32)
+    z = rng.uniform(-0.7, 0.7, n).astype(np.float32)
+    x[:200] = np.linspace(-1.0, 4.0, 200, dtype=np.float32)      # on grid lines and the ends
+    y[200:400] = np.float32(1.0); z[400:600] = np.float32(-0.5)
+    sc = [np.float32(v) for v in T3_SCALE]
+    out = {}
+    for rd, key in ((0, "o_get"), (1, "o_rounddown")):
+        o = np.zeros(n, np.float32)
+        ol.ref.ref_table3d_get_batch(w, h, dp, *sc, ol.p(d3), rd, ol.p(x), ol.p(y), ol.p(z), n, ol.p(o))
+        out[key] = o
+    np.savez_compressed(os.path.join(HERE, "tables3d.npz"), d3=d3, x=x, y=y, z=z,
+                        scale=np.array(T3_SCALE, np.float32), **out)
+    print("table_3D fixture written")
+
+
 def main():
     if ol.ref is None:
         sys.exit("oracle/_ref/libnbl_ref.so is not built (needs /root/reference): make -C oracle")
@@ -117,6 +143,7 @@ def main():
     geometry_fixture("soup", random_soup(600, 2), 6000, 2, [0, 1, -123])
     geometry_fixture("lines", scenes.line_trench(spacing=40.0), 6000, 3, [0, 1, -123])
     table_fixture()
+    table3d_fixture()
 
 
 if __name__ == "__main__":

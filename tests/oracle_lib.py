@@ -41,6 +41,9 @@ def _load_orc():
         "orc_materials_max_energy": (_F, [_P]),
         "orc_table1d_get_batch": (None, [_P, _I32, _I32, _P, _U32, _P]),
         "orc_table2d_get_batch": (None, [_P, _I32, _I32, _I32, _P, _P, _U32, _P]),
+        "orc_penn_prepare": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+        "orc_table3d_get_batch": (None, [_P, _I32, _I32, _I32, _P, _P, _P, _U32, _P]),
+        "orc_penn_plasmon_x": (None, [_F, _U32, _U64, _P]),
         "orc_physics_batch": (C.c_int, [_P, _P, _I32, _P, _P, _P, _U32, _P, _P, _P, _P, _P, _P, _P]),
         "orc_rng_seed": (None, [_U64, _U64, _U32, _P]),
         "orc_rng_uniform_batch": (None, [_P, _U32, _U32, _P]),
@@ -79,6 +82,8 @@ def _load_ref():
         "ref_intersect_ray_batch": (None, [_P, _P, _P, _U32, _P]),
         "ref_table1d_get_batch": (None, [_I32, _F, _F, _P, _P, _U32, _P]),
         "ref_table2d_get_batch": (None, [_I32, _I32, _F, _F, _F, _F, _P, _I32, _P, _P, _U32, _P]),
+        "ref_table3d_get_batch": (None, [_I32, _I32, _I32, _F, _F, _F, _F, _F, _F, _P, _I32, _P, _P, _P, _U32,
+                                         _P]),
         "ref_tri_file_load": (C.c_int64, [C.c_char_p, _P]),
         "ref_pri_file_load": (C.c_int64, [C.c_char_p, _P, _P, _F, C.c_int, C.c_uint64, _P, _P]),
     }
