@@ -13,8 +13,12 @@ the ICDF tables and once with the learned BF16 sampler trained on 1e8
 samples of the same tables; primaries/s of both, yields, and the sampler's
 KS distance per energy bin.
 
+Third part (unless --no-penn): the rough-surface workload with Si under the
+full Penn inelastic model (the reference's default physics) instead of
+Kieft-Bosch: primaries/s and yields of one launch after a warm-up.
+
 Output: one JSON line (metric, value, roofline, cpu_baseline, counters,
-learned_vs_table).
+learned_vs_table, full_penn).
 `--gpus N` with N > 1 exits: this build measures one GPU.
 """
 import argparse
@@ -39,6 +43,7 @@ WORKLOAD_TEXT = {
              "synthetic Si, 300 eV uniform over 1 um^2, 1e7 primaries",
     "stripes8": "rough surface in 8 material stripes, 533502 triangles, 500 eV uniform over 1 um^2, 1e7 primaries",
     "rough500": "rough surface, one stripe material, 500 eV uniform over 1 um^2, 1e7 primaries",
+    "rough_penn": "rough surface as 'rough' with Si under the full Penn inelastic model, 300 eV, 1e7 primaries",
     "lines": "Si + 5 PMMA lines (32 nm wide, 64 nm pitch, 50 nm tall), 10004 triangles, 500 eV Gaussian sigma 1 nm, "
              "256x64 pixels x 1000 primaries",
     "scan3": "lines geometry with 3 materials, 4096 pixels x Poisson(200) primaries, 200-2000 eV",
@@ -110,6 +115,30 @@ def learned_vs_table(primaries, seed, train_samples):
                          "energy_range_eV": [rep["e_min"], rep["e_max"]]}}
 
 
+def full_penn(tris, specs, beam, steps):
+    """The same geometry and beam with every material under full Penn."""
+    import nebula_amd as na
+    from nebula_amd import synth
+    from nebula_amd.driver import Driver
+    m = na.Materials([synth.penn(s) if s.inelastic != "penn" else s for s in specs])
+    g = na.Geometry(tris)
+    with Driver(g, m, capacity=1024, seed=31) as drv:
+        drv.run_persistent(dict(beam, seed=900))                          # warm-up
+        ms, tot = [], None
+        t1 = time.perf_counter()
+        for s in range(steps):
+            t = drv.run_persistent(dict(beam, seed=901 + s))
+            ms.append(t["kernel_ms"])
+            tot = t if tot is None else {k: tot[k] + t[k] for k in t}
+        wall = time.perf_counter() - t1
+    n = tot["primaries"]
+    print(f"full Penn: {n / wall:.4g} primaries/s, SE {tot['detected_se'] / n:.4f} BSE {tot['detected_bse'] / n:.4f}",
+          file=sys.stderr, flush=True)
+    return {"workload": "same geometry and beam, materials under the full Penn inelastic model",
+            "primaries_per_s": n / wall, "avg_kernel_ms": float(np.mean(ms)), "events_per_s": tot["events"] / wall,
+            "se": tot["detected_se"] / n, "bse": tot["detected_bse"] / n, "stopped": tot["stopped"], "steps": steps}
+
+
 def cpu_baseline(tris, specs, beam, n_sample, time_limit, threads):
     """The oracle's cpu_mt restatement on a bounded sample of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -136,6 +165,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-table sampler comparison")
+    ap.add_argument("--no-penn", action="store_true", help="skip the full Penn run of the same workload")
     ap.add_argument("--learned-primaries", type=int, default=10_000_000)
     ap.add_argument("--train-samples", type=int, default=100_000_000)
     ap.add_argument("--block-size", type=int, default=0)
@@ -215,6 +245,10 @@ def main():
                "bse_yield": st["detected_bse"] / max(st["primaries"], 1)}
         print(f"cpu baseline: {cpu['value']:.0f} primaries/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
 
+    penn = None
+    if not args.no_penn and all(s.inelastic == "kieft" for s in specs):
+        penn = full_penn(tris, specs, base_beam, max(1, min(args.steps, 2)))
+
     learned = None
     if not args.no_learned:
         learned = learned_vs_table(args.learned_primaries, 5, args.train_samples)
@@ -247,6 +281,7 @@ def main():
         "counters_per_step": per_launch,
         "yields": {"se": totals["detected_se"] / totals["primaries"], "bse": totals["detected_bse"] / totals["primaries"]},
         "learned_vs_table": learned,
+        "full_penn": penn,
     }
     print(json.dumps(out), flush=True)
 

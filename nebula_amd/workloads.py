@@ -64,7 +64,14 @@ def rough500(n=10_000_000):
     return tris, [synth.stripe_materials(8)[0]], uniform_beam(500.0, n, (500.0, 500.0), _top(tris) + 5.0, seed=0)
 
 
-BY_NAME = {"cube": cube, "lines": lines, "rough": rough, "scan3": scan3, "stripes8": stripes8, "rough500": rough500}
+def rough_penn(n=10_000_000):
+    """The rough-surface workload with Si under the full Penn inelastic model."""
+    tris, _, beam = rough(n)
+    return tris, [synth.SI_PENN], beam
+
+
+BY_NAME = {"cube": cube, "lines": lines, "rough": rough, "scan3": scan3, "stripes8": stripes8, "rough500": rough500,
+           "rough_penn": rough_penn}
 
 
 def sample_primaries(beam, n, seed=0):
