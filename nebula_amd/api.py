@@ -83,33 +83,46 @@ def octree_linearize(triangles):
 class PreparedMaterial:
     """A material's tables after the reference's create() post-processing
     (nbl_kieft_prepare, then nbl_penn_prepare for a full Penn material),
-    held in numpy arrays; .desc points at them."""
+    held in numpy arrays; .desc points at them.  The raw tables come from a
+    synthetic MaterialSpec or, as KieftRaw / PennRaw structs, from a material
+    file (`keep` holds whatever owns their arrays)."""
 
-    def __init__(self, spec: synth.MaterialSpec):
-        raw = synth.raw_material(spec)
+    def __init__(self, spec: synth.MaterialSpec | None = None, kieft_raw=None, penn_raw=None, keep=None,
+                 name=None):
+        if spec is not None:
+            if spec.inelastic not in ("kieft", "penn"):
+                raise ValueError(f"unknown inelastic model {spec.inelastic!r}")
+            keep = synth.raw_material(spec)
+            kieft_raw = keep.raw_struct(L.KieftRaw)
+            penn_raw = keep.penn_struct(L.PennRaw) if spec.inelastic == "penn" else None
+        if kieft_raw is None:
+            raise ValueError("PreparedMaterial needs a spec or raw tables")
         self.spec = spec
-        nk, np_ = synth.N_K, synth.N_P
+        self.name = name if name is not None else (spec.name if spec is not None else "")
+        self._raw = keep
+        nk, np_ = kieft_raw.n_k, kieft_raw.n_p
         self.el_imfp = np.zeros(nk, np.float32)
         self.el_icdf = np.zeros((nk, np_), np.float32)
         self.in_imfp = np.zeros(nk, np.float32)
         self.in_w0 = np.zeros((nk, np_), np.float32)
         self.ion = np.zeros((nk, np_), np.float32)
-        self.raw_struct = raw.raw_struct(L.KieftRaw)
-        self._raw = raw
+        self.raw_struct = kieft_raw
         self.desc = L.MaterialDesc()
         call("nbl_kieft_prepare", C.byref(self.raw_struct), ptr(self.el_imfp), ptr(self.el_icdf),
              ptr(self.in_imfp), ptr(self.in_w0), ptr(self.ion), C.byref(self.desc))
-        if spec.inelastic == "penn":
-            n = synth.PENN_N
-            self.in_imfp = np.zeros(nk, np.float32)
-            self.in_w0 = np.zeros((nk, np_), np.float32)
-            self.penn_q = np.zeros((n, n, n), np.float32)
-            self.penn_ion = np.zeros((n, n, n), np.float32)
-            self.penn_struct = raw.penn_struct(L.PennRaw)
+        if penn_raw is not None:
+            r = penn_raw
+            self.in_imfp = np.zeros(r.n_k, np.float32)
+            self.in_w0 = np.zeros((r.n_k, r.n_p), np.float32)
+            self.penn_q = np.zeros((r.q_nk, r.q_nw, r.q_np), np.float32)
+            self.penn_ion = np.zeros((r.i_nk, r.i_nw, r.i_np), np.float32)
+            self.penn_struct = penn_raw
             call("nbl_penn_prepare", C.byref(self.penn_struct), ptr(self.in_imfp), ptr(self.in_w0),
                  ptr(self.penn_q), ptr(self.penn_ion), C.byref(self.desc))
-        elif spec.inelastic != "kieft":
-            raise ValueError(f"unknown inelastic model {spec.inelastic!r}")
+
+    @property
+    def inelastic(self) -> str:
+        return "penn" if self.desc.inelastic_model == L.INELASTIC_PENN else "kieft"
 
 
 _PREP: dict = {}
@@ -125,7 +138,7 @@ class Materials:
     """gpu_material_manager: the listed materials uploaded to the device."""
 
     def __init__(self, specs):
-        self.prepared = [prepared(s) for s in specs]
+        self.prepared = [s if isinstance(s, PreparedMaterial) else prepared(s) for s in specs]
         self.descs = (L.MaterialDesc * len(self.prepared))(*[p.desc for p in self.prepared])
         h = C.c_void_p()
         call("nbl_materials_create", self.descs, len(self.prepared), C.byref(h))
@@ -244,7 +257,7 @@ class Sampler:
         material or stopped (gpu_driver.inl:420-428), so the transport never
         draws from those cells; a bin entirely below it gives NaN."""
         rng = np.random.default_rng(seed)
-        floor = float(self.materials.prepared[material].spec.barrier)
+        floor = float(self.materials.prepared[material].desc.barrier)
         out = []
         for lo, hi in zip(edges[:-1], edges[1:]):
             lo = max(lo, floor)

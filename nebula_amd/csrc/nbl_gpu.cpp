@@ -12,9 +12,13 @@
 //   --batch-factor      multiplication factor for batch size   [0.9]
 //   --seed              random seed                            [2026365851 = 0x78c7e39b]
 //   --sort-primaries    sort primaries by position first       [0]
+//   --inelastic         kieft | penn | auto                    [auto]
+//       (the reference fixes its inelastic model at compile time, full Penn by
+//       default; auto = penn when every material file holds full Penn tables)
 //
-// Materials: the reference reads cstool HDF5 files, and HDF5 is not part of
-// this build, so a material is a binary .nblmat file holding the same raw
+// Materials: the reference's HDF5 material files (cstool output), read
+// through libnbl_mat.so (loaded from this program's directory on first use,
+// include/nbl_mat.h), or binary .nblmat files holding the Kieft-Bosch raw
 // tables (the nbl_kieft_raw fields), little-endian:
 //   char magic[8] = "NBLMAT1\0"; uint32 n_k, n_p, n_outer_shells, 0;
 //   float64 k_min, k_max, p_min, p_max, effective_a_grams;
@@ -43,7 +47,11 @@
 #include <string>
 #include <vector>
 
+#include <dlfcn.h>
+#include <unistd.h>
+
 #include "../../include/nbl_hip.h"
+#include "../../include/nbl_mat.h"
 
 namespace {
 
@@ -95,13 +103,80 @@ struct options {
     }
 };
 
-// One .nblmat file with its arrays kept alive for nbl_kieft_prepare.
+// libnbl_mat.so, opened on first use from the directory of this program
+struct mat_api {
+    void* so = nullptr;
+    decltype(&nbl_mat_open) open = nullptr;
+    decltype(&nbl_mat_close) close = nullptr;
+    decltype(&nbl_mat_exists) exists = nullptr;
+    decltype(&nbl_mat_kieft_raw) kieft_raw = nullptr;
+    decltype(&nbl_mat_penn_raw) penn_raw = nullptr;
+    decltype(&nbl_mat_last_error) last_error = nullptr;
+    static mat_api& get()
+    {
+        static mat_api a;
+        if (a.so) return a;
+        char exe[4096] = {};
+        const ssize_t n = readlink("/proc/self/exe", exe, sizeof exe - 1);
+        std::string dir = n > 0 ? std::string(exe, (size_t)n) : std::string();
+        dir = dir.substr(0, dir.find_last_of('/') + 1);
+        a.so = dlopen((dir + "libnbl_mat.so").c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (!a.so) throw std::runtime_error(std::string("HDF5 material files need libnbl_mat.so: ") + dlerror());
+        auto sym = [&](auto& f, const char* name) {
+            f = reinterpret_cast<std::remove_reference_t<decltype(f)>>(dlsym(a.so, name));
+            if (!f) throw std::runtime_error(std::string("libnbl_mat.so lacks ") + name);
+        };
+        sym(a.open, "nbl_mat_open"); sym(a.close, "nbl_mat_close"); sym(a.exists, "nbl_mat_exists");
+        sym(a.kieft_raw, "nbl_mat_kieft_raw"); sym(a.penn_raw, "nbl_mat_penn_raw");
+        sym(a.last_error, "nbl_mat_last_error");
+        return a;
+    }
+    void check(int rc, const char* what) const
+    {
+        if (rc != NBL_OK) throw std::runtime_error(std::string(what) + ": " + last_error());
+    }
+};
+
+bool is_hdf5(const std::string& path)
+{
+    std::ifstream f(path, std::ios::binary);
+    char sig[8] = {};
+    f.read(sig, 8);
+    return f && std::memcmp(sig, "\x89HDF\r\n\x1a\n", 8) == 0;
+}
+
+// One material file (HDF5 or .nblmat) with its raw tables kept alive for the
+// table preparation.
 struct material_file {
     std::vector<float> el_imfp, el_cos, in_imfp, in_w0, binding;
     std::vector<double> outer;
     nbl_kieft_raw raw{};
+    nbl_matfile h5 = nullptr;          // HDF5 file: owns the raw arrays
+    bool has_penn = false;
+    nbl_penn_raw penn{};
+    material_file(const material_file&) = delete;
+    material_file(material_file&& o) noexcept
+        : el_imfp(std::move(o.el_imfp)), el_cos(std::move(o.el_cos)), in_imfp(std::move(o.in_imfp)),
+          in_w0(std::move(o.in_w0)), binding(std::move(o.binding)), outer(std::move(o.outer)), raw(o.raw), h5(o.h5),
+          has_penn(o.has_penn), penn(o.penn)
+    {
+        o.h5 = nullptr;
+    }
+    ~material_file() { if (h5) mat_api::get().close(h5); }
     explicit material_file(const std::string& path)
     {
+        if (is_hdf5(path)) {
+            const mat_api& m = mat_api::get();
+            m.check(m.open(path.c_str(), &h5), "nbl_mat_open");
+            m.check(m.kieft_raw(h5, &raw), "nbl_mat_kieft_raw");
+            int32_t e = 0;
+            m.check(m.exists(h5, "/full_penn", &e), "nbl_mat_exists");
+            if (e) {
+                m.check(m.penn_raw(h5, &penn), "nbl_mat_penn_raw");
+                has_penn = true;
+            }
+            return;
+        }
         std::ifstream f(path, std::ios::binary);
         if (!f) throw std::runtime_error("cannot open material file '" + path + "'");
         char magic[8];
@@ -133,17 +208,23 @@ struct material_file {
     }
 };
 
-// Prepared tables of one material (nbl_kieft_prepare output) and its descriptor.
+// Prepared tables of one material (nbl_kieft_prepare, then nbl_penn_prepare
+// for full Penn) and its descriptor.
 struct prepared_material {
-    std::vector<float> el_imfp, el_icdf, in_imfp, in_w0, ion;
+    std::vector<float> el_imfp, el_icdf, in_imfp, in_w0, ion, p_imfp, p_omega, p_q, p_ion;
     nbl_material_desc desc{};
-    explicit prepared_material(const material_file& m)
+    prepared_material(const material_file& m, bool penn)
     {
         const size_t nk = m.raw.n_k, np = m.raw.n_p;
         el_imfp.resize(nk); in_imfp.resize(nk);
         el_icdf.resize(nk * np); in_w0.resize(nk * np); ion.resize(nk * np);
         check(nbl_kieft_prepare(&m.raw, el_imfp.data(), el_icdf.data(), in_imfp.data(), in_w0.data(), ion.data(), &desc),
               "nbl_kieft_prepare");
+        if (!penn) return;
+        const nbl_penn_raw& r = m.penn;
+        p_imfp.resize((size_t)r.n_k); p_omega.resize((size_t)r.n_k * r.n_p);
+        p_q.resize((size_t)r.q_nk * r.q_nw * r.q_np); p_ion.resize((size_t)r.i_nk * r.i_nw * r.i_np);
+        check(nbl_penn_prepare(&r, p_imfp.data(), p_omega.data(), p_q.data(), p_ion.data(), &desc), "nbl_penn_prepare");
     }
 };
 
@@ -194,9 +275,7 @@ int run(int argc, char** argv)
 {
     int maj = 0, min = 0, pat = 0;
     nbl_version(&maj, &min, &pat);
-    std::clog << "This is nbl_gpu (libnbl_hip " << maj << "." << min << "." << pat << ")\n\n"
-              << "Physics models:\n  Kieft-Bosch elastic (Mott + phonon), Kieft-Bosch inelastic, boundary with "
-                 "quantum-mechanical transmission and refraction\n\n";
+    std::clog << "This is nbl_gpu (libnbl_hip " << maj << "." << min << "." << pat << ")\n\n";
     options p;
     p.add("energy-threshold", "Lowest energy to simulate", "0");
     p.add("capacity", "Electron capacity on the GPU", "1000000");
@@ -204,6 +283,7 @@ int run(int argc, char** argv)
     p.add("batch-factor", "Multiplication factor for electron batch size", "0.9");
     p.add("seed", "Random seed", std::to_string(0x78c7e39bu));
     p.add("sort-primaries", "Sort primary electrons before simulation", "0");
+    p.add("inelastic", "Inelastic model: kieft, penn or auto", "auto");
     p.parse(argc, argv);
     const float energy_threshold = p.get<float>("energy-threshold");
     const uint32_t capacity = p.get<uint32_t>("capacity");
@@ -211,6 +291,11 @@ int run(int argc, char** argv)
     const float batch_factor = p.get<float>("batch-factor");
     const uint32_t seed = p.get<uint32_t>("seed");
     const bool sort_primaries = p.get<bool>("sort-primaries");
+    const std::string inelastic = p.get<std::string>("inelastic");
+    if (inelastic != "kieft" && inelastic != "penn" && inelastic != "auto") {
+        std::clog << "Error: --inelastic must be kieft, penn or auto\n";
+        return 1;
+    }
     const auto& pos = p.positional;
     if (pos.size() < 3 || capacity == 0 || prescan_size == 0 || !(batch_factor > 0)) {
         p.usage(std::clog, argv[0]);
@@ -252,11 +337,21 @@ int run(int argc, char** argv)
     std::vector<prepared_material> prep;
     files.reserve(pos.size() - 2);
     prep.reserve(pos.size() - 2);
+    bool all_penn = true;
     for (size_t k = 2; k < pos.size(); ++k) {
         files.emplace_back(pos[k]);
-        prep.emplace_back(files.back());
+        all_penn = all_penn && files.back().has_penn;
+        if (inelastic == "penn" && !files.back().has_penn)
+            throw std::runtime_error("Full Penn model not found in material file " + pos[k]);
+    }
+    const bool penn = inelastic == "penn" || (inelastic == "auto" && all_penn);
+    for (size_t k = 2; k < pos.size(); ++k) {
+        prep.emplace_back(files[k - 2], penn);
         std::clog << "  Material " << (k - 2) << ": " << pos[k] << "\n";
     }
+    std::clog << "Physics models:\n  Kieft-Bosch elastic (Mott + phonon), "
+              << (penn ? "full Penn inelastic" : "Kieft-Bosch inelastic")
+              << ", boundary with quantum-mechanical transmission and refraction\n";
     std::vector<nbl_material_desc> descs;
     for (auto& m : prep) descs.push_back(m.desc);
     nbl_materials mats = nullptr;

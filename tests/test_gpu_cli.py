@@ -62,3 +62,56 @@ def test_cli_usage_and_errors(tmp_path):
     r = subprocess.run([CLI, str(tmp_path / "g.tri"), str(tmp_path / "p.pri"), str(tmp_path / "si.nblmat")],
                        capture_output=True, timeout=120)
     assert r.returncode == 1 and b"not enough materials" in r.stderr
+
+
+def _cli(*args, timeout=300):
+    r = subprocess.run([CLI, *map(str, args)], capture_output=True, timeout=timeout)
+    err = r.stderr.decode(errors="replace")
+    assert r.returncode == 0, err
+    return r.stdout, err
+
+
+def test_cli_reads_hdf5_materials(tmp_path):
+    """The reference's HDF5 material layout (written by nbl_mat_write) in place
+    of .nblmat: with --inelastic kieft the detected records are the same as
+    from the .nblmat file (same tables bit for bit, same seed); by default the
+    file's full Penn tables are used, and the yields match the persistent
+    kernel's full Penn run."""
+    print("[gpu] nbl_gpu with HDF5 materials", flush=True)
+    pytest.importorskip("nebula_amd.matfile")
+    from nebula_amd import matfile
+    if not os.path.exists(os.path.join(os.path.dirname(L.LIB_PATH), "libnbl_mat.so")):
+        pytest.skip("libnbl_mat.so not built")
+    tris = scenes.cube_slab()
+    a = tris["r0"].astype(np.float64)
+    files.write_tri_file(tmp_path / "g.tri", tris["material_in"], tris["material_out"], a, a + tris["e1"],
+                         a + tris["e2"])
+    n = 20_000
+    prim = runs.pencil_primaries(n, 1000.0)
+    pix = np.stack([np.arange(n) % 100, np.arange(n) // 100], axis=1).astype(np.int32)
+    files.write_pri_file(tmp_path / "p.pri", prim, pix)
+    synth.write_material_file(synth.SI, tmp_path / "si.nblmat")
+    matfile.write_material(tmp_path / "si.mat", synth.SI)
+    common = ["--capacity", "200000", "--seed", "7", tmp_path / "g.tri", tmp_path / "p.pri"]
+    out_nbl, _ = _cli(*common, tmp_path / "si.nblmat")
+    out_h5, err = _cli("--inelastic", "kieft", *common, tmp_path / "si.mat")
+    assert "Kieft-Bosch inelastic" in err
+    ra, rb = files.read_records(out_nbl), files.read_records(out_h5)
+    key = lambda r: np.lexsort([r["kin_energy"], r["pos"][:, 0], r["pos"][:, 1], r["pixel"][:, 0], r["pixel"][:, 1]])
+    assert len(ra) == len(rb) and len(ra) > 0
+    assert ra[key(ra)].tobytes() == rb[key(rb)].tobytes()
+    out_p, err = _cli(*common, tmp_path / "si.mat")
+    assert "full Penn inelastic" in err
+    rec = files.read_records(out_p)
+    se, bse = int((rec["kin_energy"] < 50).sum()), int((rec["kin_energy"] >= 50).sum())
+    g = na.Geometry(tris); m = na.Materials([synth.SI_PENN])
+    with Driver(g, m, capacity=1 << 16, seed=3) as drv:
+        t = drv.run_persistent(raster_scan(np.zeros(1), np.zeros(1), 5.0, 1000.0, 200_000, 0.0, seed=5))
+    for k_cli, k_p in ((se, t["detected_se"]), (bse, t["detected_bse"])):
+        z = runs.yield_z(k_cli, n, k_p, t["primaries"])
+        print(f"  cli (Penn, HDF5) {k_cli / n:.4f} persistent {k_p / t['primaries']:.4f} z {z:+.2f}")
+        assert abs(z) < 4
+    # a .nblmat file has no full Penn tables
+    r = subprocess.run([CLI, "--inelastic", "penn", *map(str, common), str(tmp_path / "si.nblmat")],
+                       capture_output=True, timeout=120)
+    assert r.returncode == 1 and b"Full Penn model not found" in r.stderr
