@@ -51,6 +51,26 @@ WORKLOAD_TEXT = {
 }
 
 
+def profiled_traffic(workload, primaries, kernel_ms):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    copy-kernel calibrated) for this workload, from the committed profile
+    summary; None unless the workload and launch size match and this run's
+    kernel time is within 15 % of the profiled one (same kernel build)."""
+    summary = {"rough": "k_persistent_rough_surface_300eV"}.get(workload)
+    if summary is None:
+        return None, None
+    path = os.path.join(ROOT, "profiles", summary + ".traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        prof = json.load(f)
+    if prof.get("workload") != workload or prof.get("primaries_per_launch") != primaries:
+        return None, None
+    if not abs(kernel_ms / prof["avg_kernel_ms"] - 1) < 0.15:
+        return None, None
+    return float(prof["traffic_bytes"]), os.path.relpath(path, ROOT)
+
+
 def algorithmic_bytes(t):
     return (BYTES["event"] * t["events"] + BYTES["lookup_1d"] * t["lookups_1d"] +
             BYTES["lookup_2d"] * t["lookups_2d"] + BYTES["rounddown"] * t["rounddowns"] +
@@ -230,6 +250,7 @@ def main():
     avg_kernel_ms = float(np.mean(kernel_ms))
     per_launch = {k: v / args.steps for k, v in totals.items()}
     ach_gbps = algorithmic_bytes(per_launch) / (avg_kernel_ms * 1e-3) / 1e9
+    traffic, traffic_src = profiled_traffic(args.workload, n_step, avg_kernel_ms)
 
     cpu = None
     if not args.no_cpu:
@@ -272,7 +293,7 @@ def main():
                    "sampler": "table", "kernel": "persistent",
                    "launch": {k: v for k, v in opts.items()}},
         "roofline": {"bound": "hbm", "achieved": ach_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": ach_gbps / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": ach_gbps / HBM_PEAK_GBPS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_persistent", "avg_kernel_ms": avg_kernel_ms, "copy_kernel_GBps": copy_gbps,
                      "algorithmic_bytes_per_launch": algorithmic_bytes(per_launch)},
         "cpu_baseline": cpu,

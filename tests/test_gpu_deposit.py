@@ -33,14 +33,16 @@ def _oracle(og, om, n, seed):
     return grid, tot.value, cnt.value, {f: getattr(st, f) for f, _ in ol.RunStats._fields_}
 
 
-def test_deposit_tally_matches_oracle():
+@pytest.mark.parametrize("spec", [synth.SI, synth.SI_PENN], ids=lambda s: s.inelastic)
+def test_deposit_tally_matches_oracle(spec):
     """Per-primary means over independent runs: 10 oracle runs of 3000
     primaries, 20 GPU launches of 20000; every depth bin, the total and the
     deposit count compared by Welch z (|z| < 4 for each of ~15 quantities,
-    a family-wise level of about 1e-3)."""
-    print("[gpu] energy deposition vs oracle", flush=True)
+    a family-wise level of about 1e-3).  Kieft-Bosch and full Penn Si (the
+    kernel's deposit variant with each inelastic model)."""
+    print(f"[gpu] energy deposition vs oracle ({spec.inelastic})", flush=True)
     tris = scenes.cube_slab()
-    g = na.Geometry(tris); m = na.Materials([synth.SI])
+    g = na.Geometry(tris); m = na.Materials([spec])
     og = ol.OrcGeometry(tris); om = ol.OrcMaterials([m.prepared[0].desc])
     n_cpu, n_gpu = 3000, 20_000
     cpu = []
