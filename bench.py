@@ -95,12 +95,14 @@ def learned_vs_table(primaries, seed, train_samples):
     # tables' lower end (1 eV, below every barrier) to above the beam energy
     smp = na.Sampler(m, samples=train_samples, e_min=0.0, e_max=1.2 * e_beam, seed=seed)
     train_s = time.time() - t0
+    # the elastic network alone (inelastic draws from the tables)
+    smp_el = na.Sampler(m, samples=train_samples, e_min=0.0, e_max=1.2 * e_beam, seed=seed, kinds="elastic")
     rep = smp.report()
     edges = np.geomspace(5.0, 0.95 * e_beam, 9)
     ks = {k: [smp.ks_per_bin(kind, mat, edges, n=20000, seed=mat) for mat in range(len(specs))]
           for k, kind in (("elastic", 0), ("inelastic", 1))}
     out = {}
-    for name, sampler in (("table", None), ("learned", smp)):
+    for name, sampler in (("table", None), ("learned", smp), ("learned_elastic", smp_el)):
         with Driver(g, m, capacity=1024, seed=77) as drv:
             drv.set_sampler(sampler)
             drv.run_persistent(dict(beam, seed=1))                       # warm-up
@@ -113,18 +115,27 @@ def learned_vs_table(primaries, seed, train_samples):
                      "counts": (t["detected_se"], t["detected_bse"], n)}
         print(f"{name}: {n / wall:.4g} primaries/s, SE {out[name]['se']:.4f} BSE {out[name]['bse']:.4f}",
               file=sys.stderr, flush=True)
-    z = {}
-    for i, key in enumerate(("se", "bse")):
-        (a, na_), (b, nb_) = (out["table"]["counts"][i], out["table"]["counts"][2]), \
-                             (out["learned"]["counts"][i], out["learned"]["counts"][2])
-        # Poisson counting error of each yield
-        z[key] = (b / nb_ - a / na_) / np.hypot(np.sqrt(max(a, 1)) / na_, np.sqrt(max(b, 1)) / nb_)
+    def yield_z(name):
+        z = {}
+        for i, key in enumerate(("se", "bse")):
+            (a, na_), (b, nb_) = (out["table"]["counts"][i], out["table"]["counts"][2]), \
+                                 (out[name]["counts"][i], out[name]["counts"][2])
+            # Poisson counting error of each yield
+            z[key] = (b / nb_ - a / na_) / np.hypot(np.sqrt(max(a, 1)) / na_, np.sqrt(max(b, 1)) / nb_)
+        return z
+    z, z_el = yield_z("learned"), yield_z("learned_elastic")
     for v in out.values():
         v.pop("counts")
     return {"workload": WORKLOAD_TEXT["stripes8"].replace("1e7", f"{primaries:.0e}"),
             "table": out["table"], "learned": out["learned"],
             "speedup_learned_vs_table": out["learned"]["primaries_per_s"] / out["table"]["primaries_per_s"],
             "yield_z": z, "yields_equal_3sigma": bool(max(abs(z["se"]), abs(z["bse"])) < 3),
+            "learned_elastic_only": {**out["learned_elastic"],
+                                     "speedup_vs_table": out["learned_elastic"]["primaries_per_s"] /
+                                     out["table"]["primaries_per_s"],
+                                     "yield_z": z_el,
+                                     "yields_equal_3sigma": bool(max(abs(z_el["se"]), abs(z_el["bse"])) < 3),
+                                     "note": "elastic cos(theta) from the network, inelastic omega' from the tables"},
             "ks_per_bin": {"energy_edges_eV": [float(e) for e in edges],
                            "note": "per material (rows) and energy bin; table cells below the Fermi level excluded",
                            **{k: [[None if not np.isfinite(x) else round(float(x), 5) for x in row] for row in v]

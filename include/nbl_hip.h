@@ -464,8 +464,12 @@ typedef struct {
     float lr;                /* Adam learning rate (cosine decay); 0 = 4e-3       */
     float e_min, e_max;      /* kinetic-energy range of the samples, eV; 0 = the tables' */
     uint64_t seed;
-    uint32_t reserved[4];
+    uint32_t kinds;          /* tables the networks replace: bit NBL_SAMPLER_ELASTIC,
+                                bit NBL_SAMPLER_INELASTIC; 0 = both.  The others stay
+                                tables in the transport (and are not trained). */
+    uint32_t reserved[3];
 } nbl_sampler_opts;
+#define NBL_SAMPLER_KIND_BIT(k) (1u << (k))
 typedef struct {
     double loss_elastic, loss_inelastic;   /* final mean squared error, standardised targets */
     double train_ms;
@@ -473,7 +477,8 @@ typedef struct {
     double e_min, e_max;                   /* energy range trained on, eV */
 } nbl_sampler_report;
 
-/* Trains both networks for the materials of m (1..64 materials). */
+/* Trains the networks of opts->kinds (default both) for the materials of m
+ * (1..64 materials); the report's loss of a kind not trained is 0. */
 int nbl_sampler_create(nbl_materials m, const nbl_sampler_opts* opts, nbl_sampler* out);
 int nbl_sampler_destroy(nbl_sampler s);
 int nbl_sampler_get_report(nbl_sampler s, nbl_sampler_report* r);
@@ -481,7 +486,7 @@ int nbl_sampler_get_report(nbl_sampler s, nbl_sampler_report* r);
  * feature-0 scale, output shift, output scale. */
 int nbl_sampler_get_params(nbl_sampler s, int kind, float* params, uint32_t n, float* consts);
 /* The network's value for n samples (host arrays), through the matrix-core
- * path the transport uses: cos(theta) or log(omega'). */
+ * path the transport uses: cos(theta) or log(omega') (trained kinds only). */
 int nbl_sampler_eval_batch(nbl_sampler s, int kind, const int32_t* mat, const float* energy, const float* u,
     uint32_t n, float* out);
 /* Known-answer test of the BF16 matrix-core operand layout (one 16x16x16

@@ -175,7 +175,7 @@ INT2_DTYPE = np.dtype([("x", np.int32), ("y", np.int32)])
 class SamplerOpts(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("batch", C.c_uint32), ("lr", C.c_float),
                 ("e_min", C.c_float), ("e_max", C.c_float), ("seed", C.c_uint64),
-                ("reserved", C.c_uint32 * 4)]
+                ("kinds", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class SamplerReport(C.Structure):

@@ -200,17 +200,21 @@ def ks_distance(a, b) -> float:
 
 
 class Sampler:
-    """Learned ICDF sampler of a material set: two BF16 MLPs (elastic
-    cos(theta), inelastic log omega') evaluated on the matrix cores, trained
-    at construction on samples drawn from the material tables."""
+    """Learned ICDF sampler of a material set: BF16 MLPs (elastic cos(theta),
+    inelastic log omega') evaluated on the matrix cores, trained at
+    construction on samples drawn from the material tables.  kinds selects
+    which tables the networks replace ("both", "elastic", "inelastic"); the
+    others stay tables in the transport."""
 
     KINDS = {L.SAMPLER_ELASTIC: L.TAB_ELASTIC_ICDF, L.SAMPLER_INELASTIC: L.TAB_INELASTIC_W0_ICDF}
+    KIND_MASKS = {"both": 3, "elastic": 1 << L.SAMPLER_ELASTIC, "inelastic": 1 << L.SAMPLER_INELASTIC}
 
     def __init__(self, materials: Materials, samples=100_000_000, batch=16384, lr=4e-3, e_min=0.0, e_max=0.0,
-                 seed=1):
+                 seed=1, kinds="both"):
         self.materials = materials              # the tables must outlive the sampler
+        self.kinds = kinds
         o = L.SamplerOpts(samples=int(samples), batch=int(batch), lr=float(lr), e_min=float(e_min),
-                          e_max=float(e_max), seed=int(seed))
+                          e_max=float(e_max), seed=int(seed), kinds=self.KIND_MASKS[kinds])
         h = C.c_void_p()
         call("nbl_sampler_create", materials.h, C.byref(o), C.byref(h))
         self.h = h

@@ -67,6 +67,7 @@ struct pers_params {
     uint32_t ovf;                       // HBM stack entries per wavefront
     uint32_t refill_threshold;
     dmlp mlp[2];                        // learned sampler (elastic, inelastic) when LEARNED
+    uint32_t learned;                   // bit k: network k replaces table k (LEARNED)
     // energy-deposition tally (dep == nullptr: off)
     double* dep;                        // [dnz][dny][dnx] voxels
     double* dep_total;                  // all deposits
@@ -299,12 +300,16 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
             bool rot = false;
             scatter_state ss;
             if (LEARNED) {
+                // the networks of P.learned (wave-uniform) replace their
+                // tables; a kind left out is read from its table as usual
                 float y = 0.5f, le = 0.0f;
                 if (el || go) { y = rng.unit(); le = logf(p.e); }
-                float v = 0;
-                const float ve = mlp_eval_needed(P.mlp[MLP_KIND_ELASTIC], el, mat, le, y);
-                const float vi = mlp_eval_needed(P.mlp[MLP_KIND_INELASTIC], go, mat, le, y);
-                v = el ? ve : vi;
+                float ve = 0, vi = 0;
+                if (P.learned & (1u << MLP_KIND_ELASTIC)) ve = mlp_eval_needed(P.mlp[MLP_KIND_ELASTIC], el, mat, le, y);
+                else if (el) ve = tab2_get(P.mats[mat].el_icdf, le, y);
+                if (P.learned & (1u << MLP_KIND_INELASTIC)) vi = mlp_eval_needed(P.mlp[MLP_KIND_INELASTIC], go, mat, le, y);
+                else if (go) vi = tab2_get(P.mats[mat].in_icdf, le, y);
+                const float v = el ? ve : vi;
                 if (el || go) {
                     pend = false;
                     last = NO_TRI;
@@ -497,6 +502,7 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
         if (d->sampler) {
             const dmlp* v = sampler_views(d->sampler);
             P.mlp[0] = v[0]; P.mlp[1] = v[1];
+            P.learned = sampler_kinds(d->sampler);
             if (dg) hipLaunchKernelGGL((k_persistent<true, true, false>), grid, threads_per_block, lds, s, P);
             else hipLaunchKernelGGL((k_persistent<true, false, false>), grid, threads_per_block, lds, s, P);
         } else if (penn) {

@@ -10,4 +10,6 @@ namespace nbl {
 const dmlp* sampler_views(nbl_sampler s);
 // The material set the sampler was trained for.
 nbl_materials sampler_materials(nbl_sampler s);
+// Bit k set: network k replaces table k in the transport.
+uint32_t sampler_kinds(nbl_sampler s);
 }

@@ -314,11 +314,13 @@ struct nbl_sampler_s {
     dbuf<float> prm[2];            // FP32 parameters per network
     dbuf<bf16x4> packed[2];        // BF16 A operands per network
     dmlp view[2]{};
+    uint32_t kinds = 3;            // bit k: network k trained and used
     nbl_sampler_report report{};
 };
 
 namespace nbl {
 const dmlp* sampler_views(nbl_sampler s) { return s ? s->view : nullptr; }
+uint32_t sampler_kinds(nbl_sampler s) { return s ? s->kinds : 0u; }
 nbl_materials sampler_materials(nbl_sampler s) { return s ? s->mats : nullptr; }
 }
 
@@ -441,7 +443,10 @@ int nbl_sampler_create(nbl_materials m, const nbl_sampler_opts* opts, nbl_sample
             if (opts->batch) o.batch = opts->batch;
             if (opts->lr > 0) o.lr = opts->lr;
             o.e_min = opts->e_min; o.e_max = opts->e_max; o.seed = opts->seed;
+            o.kinds = opts->kinds;
         }
+        const uint32_t kinds = o.kinds ? o.kinds : 3u;
+        require(kinds <= 3u, "nbl_sampler_create: kinds must be a mask of the elastic and inelastic bits");
         require(o.batch >= 256 && o.batch <= (1u << 22) && o.batch % 256 == 0,
             "nbl_sampler_create: batch must be a multiple of 256 in [256, 4194304]");
         const int n_mat = materials_count(m);
@@ -460,9 +465,11 @@ int nbl_sampler_create(nbl_materials m, const nbl_sampler_opts* opts, nbl_sample
         auto s = std::make_unique<nbl_sampler_s>();
         s->mats = m;
         s->n_mat = n_mat;
+        s->kinds = kinds;
         const auto t0 = std::chrono::steady_clock::now();
-        train_network(*s, MLP_KIND_ELASTIC, o, lo, hi, s->report.loss_elastic);
-        train_network(*s, MLP_KIND_INELASTIC, o, lo, hi, s->report.loss_inelastic);
+        if (kinds & (1u << MLP_KIND_ELASTIC)) train_network(*s, MLP_KIND_ELASTIC, o, lo, hi, s->report.loss_elastic);
+        if (kinds & (1u << MLP_KIND_INELASTIC))
+            train_network(*s, MLP_KIND_INELASTIC, o, lo, hi, s->report.loss_inelastic);
         s->report.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         s->report.steps = std::max<uint64_t>(1, o.samples / o.batch);
         s->report.samples = s->report.steps * o.batch;
@@ -485,6 +492,7 @@ int nbl_sampler_get_params(nbl_sampler s, int kind, float* params, uint32_t n, f
     return guarded([&] {
         require(s != nullptr, "nbl_sampler_get_params: NULL sampler");
         require(kind == NBL_SAMPLER_ELASTIC || kind == NBL_SAMPLER_INELASTIC, "nbl_sampler_get_params: bad kind");
+        require(s->kinds & (1u << kind), "nbl_sampler_get_params: this network was not trained");
         require(n == (uint32_t)P_TOTAL, "nbl_sampler_get_params: n must be NBL_SAMPLER_PARAMS");
         if (params) s->prm[kind].download(params, P_TOTAL);
         if (consts) {
@@ -500,6 +508,7 @@ int nbl_sampler_eval_batch(nbl_sampler s, int kind, const int32_t* mat, const fl
     return guarded([&] {
         require(s != nullptr, "nbl_sampler_eval_batch: NULL sampler");
         require(kind == NBL_SAMPLER_ELASTIC || kind == NBL_SAMPLER_INELASTIC, "nbl_sampler_eval_batch: bad kind");
+        require(s->kinds & (1u << kind), "nbl_sampler_eval_batch: this network was not trained");
         if (n == 0) return;
         require(mat && energy && u && out, "nbl_sampler_eval_batch: NULL array");
         for (uint32_t i = 0; i < n; ++i) {

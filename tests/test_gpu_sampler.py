@@ -123,6 +123,34 @@ def test_persistent_learned_vs_table_yields(trained):
         assert abs(b.mean() - a.mean()) < 0.05 * max(a.mean(), 0.02)
 
 
+def test_elastic_only_sampler(trained):
+    """kinds="elastic": only the elastic network is trained and used; the
+    inelastic draws stay on the tables (the untrained network is refused),
+    and the transport's yields match the table path's within 4 sigma."""
+    print("[gpu] elastic-only sampler", flush=True)
+    m, _ = trained
+    s = na.Sampler(m, samples=30_000_000, e_min=E_LO, e_max=E_HI, seed=5, kinds="elastic")
+    r = s.report()
+    assert r["loss_inelastic"] == 0 and r["loss_elastic"] > 0
+    with pytest.raises(na.NblError):
+        s.eval(L.SAMPLER_INELASTIC, 0, np.full(4, 100.0, np.float32), np.full(4, 0.5, np.float32))
+    assert np.all(np.isfinite(s.eval(L.SAMPLER_ELASTIC, 0, np.full(4, 100.0, np.float32),
+                                     np.full(4, 0.5, np.float32))))
+    g = na.Geometry(scenes.cube_slab(material=3))
+    beam = raster_scan(np.zeros(20), np.zeros(20), 5.0, 500.0, 1000, 0.0, seed=6)
+    out = {}
+    for name, smp in (("table", None), ("learned", s)):
+        with Driver(g, m, capacity=1 << 16, seed=23) as drv:
+            drv.set_sampler(smp)
+            out[name] = drv.run_persistent(beam, per_pixel=True)
+        assert out[name]["primaries"] == 400_000 and out[name]["stopped"] == 0
+    for key in ("pixel_se", "pixel_bse"):
+        a, b = out["table"][key] / 1000.0, out["learned"][key] / 1000.0
+        z = (b.mean() - a.mean()) / (np.hypot(a.std(ddof=1), b.std(ddof=1)) / np.sqrt(len(a)))
+        print(f"  {key}: table {a.mean():.4f} learned (elastic) {b.mean():.4f} z {z:+.2f}")
+        assert abs(z) < 4
+
+
 def test_set_sampler_rejects_other_material_set(trained):
     m, s = trained
     g = na.Geometry(scenes.cube_slab())
