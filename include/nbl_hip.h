@@ -382,6 +382,17 @@ typedef struct {
     uint64_t count;          /* out: deposit events                                    */
 } nbl_deposit_grid;
 
+/* Detected electrons of a persistent run as records (the reference's output,
+ * gpu_main.cu:199-206, before the pixel lookup), in detection order: the
+ * particle at the detector and its primary's tag.  count is every detection;
+ * only the first min(count, capacity) are stored (capacity 0: count only). */
+typedef struct {
+    nbl_particle* particles; /* [capacity] out                                        */
+    uint32_t* tags;          /* [capacity] out                                        */
+    uint64_t capacity;
+    uint64_t count;          /* out                                                    */
+} nbl_detection_buffer;
+
 typedef struct {
     uint64_t primaries;      /* primaries whose cascades ended                    */
     uint64_t detected_se;    /* detected with E < 50 eV                            */
@@ -396,6 +407,7 @@ typedef struct {
     uint32_t* pixel_bse;     /* optional [n_pixels] out (may be NULL)              */
     uint32_t* hist;          /* optional [NBL_HIST_BINS] detected-energy histogram, 1-eV bins */
     nbl_deposit_grid* deposit; /* optional energy-deposition tally (may be NULL)      */
+    nbl_detection_buffer* detected; /* optional detected-electron records (may be NULL) */
 } nbl_tallies;
 
 typedef struct {
@@ -410,6 +422,12 @@ typedef struct {
 /* Runs every primary of the batch to the end of its cascade in one
  * persistent launch; per-pixel SE/BSE tallies and counters go to *out. */
 int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch,
+    const nbl_persistent_opts* opts, nbl_tallies* out);
+/* The same transport over an explicit list of n primaries with tags (the
+ * primaries of gpu_main.cu's work pool, pushed by push_to_buffer in the
+ * reference); each starts in vacuum.  Per-pixel tallies are not available
+ * (out->pixel_se and pixel_bse must be NULL): use out->detected. */
+int nbl_driver_run_persistent_list(nbl_driver d, const nbl_particle* primaries, const uint32_t* tags, uint64_t n,
     const nbl_persistent_opts* opts, nbl_tallies* out);
 
 /* ---- file formats (io/load_tri_file.cpp, io/load_pri_file.cpp), host only ---- */

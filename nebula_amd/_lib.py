@@ -146,7 +146,11 @@ class Tallies(C.Structure):
                 ("boundary_events", C.c_uint64), ("secondaries", C.c_uint64),
                 ("pending_retries", C.c_uint64), ("stopped", C.c_uint64), ("kernel_ms", C.c_double),
                 ("pixel_se", C.c_void_p), ("pixel_bse", C.c_void_p), ("hist", C.c_void_p),
-                ("deposit", C.c_void_p)]
+                ("deposit", C.c_void_p), ("detected", C.c_void_p)]
+
+
+class DetectionBuffer(C.Structure):
+    _fields_ = [("particles", C.c_void_p), ("tags", C.c_void_p), ("capacity", C.c_uint64), ("count", C.c_uint64)]
 
 
 class DepositGrid(C.Structure):
@@ -233,6 +237,7 @@ SIGNATURES = {
     "nbl_rng_seed": [_U64, _U64, _U32, _P],
     "nbl_rng_uniform_batch": [_P, _U32, _U32, _P],
     "nbl_driver_run_persistent": [_P, C.POINTER(PixelBatch), C.POINTER(PersistentOpts), C.POINTER(Tallies)],
+    "nbl_driver_run_persistent_list": [_P, _P, _P, _U64, C.POINTER(PersistentOpts), C.POINTER(Tallies)],
     "nbl_tri_file_load": [C.c_char_p, _P, _U32, C.POINTER(_U32)],
     "nbl_pri_file_load": [C.c_char_p, Vec3, Vec3, _F, _P, _P, _U64, C.POINTER(_U64), C.POINTER(PriWarnings)],
     "nbl_pri_sort_morton": [_P, _P, _U64],

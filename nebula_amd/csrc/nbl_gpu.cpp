@@ -13,6 +13,9 @@
 //   --seed              random seed                            [2026365851 = 0x78c7e39b]
 //   --sort-primaries    sort primaries by position first       [0]
 //   --inelastic         kieft | penn | auto                    [auto]
+//   --persistent        0 | 1: run the primaries through the persistent
+//                       transport kernel (one launch per 2^22 primaries)
+//                       instead of the prescan and frame loop      [0]
 //       (the reference fixes its inelastic model at compile time, full Penn by
 //       default; auto = penn when every material file holds full Penn tables)
 //
@@ -284,6 +287,7 @@ int run(int argc, char** argv)
     p.add("seed", "Random seed", std::to_string(0x78c7e39bu));
     p.add("sort-primaries", "Sort primary electrons before simulation", "0");
     p.add("inelastic", "Inelastic model: kieft, penn or auto", "auto");
+    p.add("persistent", "Use the persistent transport kernel instead of the frame loop", "0");
     p.parse(argc, argv);
     const float energy_threshold = p.get<float>("energy-threshold");
     const uint32_t capacity = p.get<uint32_t>("capacity");
@@ -292,6 +296,7 @@ int run(int argc, char** argv)
     const uint32_t seed = p.get<uint32_t>("seed");
     const bool sort_primaries = p.get<bool>("sort-primaries");
     const std::string inelastic = p.get<std::string>("inelastic");
+    const bool persistent = p.get<bool>("persistent");
     if (inelastic != "kieft" && inelastic != "penn" && inelastic != "auto") {
         std::clog << "Error: --inelastic must be kieft, penn or auto\n";
         return 1;
@@ -392,6 +397,44 @@ int run(int argc, char** argv)
 
     nbl_driver d = nullptr;
     check(nbl_driver_create(capacity, g, mats, energy_threshold, max_energy, seed, nullptr, &d), "nbl_driver_create");
+    if (persistent) {
+        // whole cascades in persistent launches; detected electrons come back
+        // as records with their primary's tag (= index), in detection order
+        const auto t0 = std::chrono::steady_clock::now();
+        record_writer out{ &pixels, {}, 0 };
+        constexpr uint64_t kChunk = 1ull << 22;
+        uint64_t launches = 0;
+        for (uint64_t o = 0; o < n_pri; o += kChunk) {
+            const uint64_t k = std::min<uint64_t>(kChunk, n_pri - o);
+            uint64_t cap_records = 4 * k;
+            for (;;) {
+                std::vector<nbl_particle> det((size_t)cap_records);
+                std::vector<uint32_t> det_tags((size_t)cap_records);
+                nbl_detection_buffer db{ det.data(), det_tags.data(), cap_records, 0 };
+                nbl_tallies t{};
+                t.detected = &db;
+                check(nbl_driver_run_persistent_list(d, prim.data() + o, tags.data() + o, k, nullptr, &t),
+                      "nbl_driver_run_persistent_list");
+                ++launches;
+                if (db.count > db.capacity) {     // more detections than room: run the chunk again
+                    cap_records = 2 * db.count;
+                    continue;
+                }
+                for (uint64_t i = 0; i < db.count; ++i) record_writer::on_detect(&det[i], det_tags[i], &out);
+                break;
+            }
+            std::clog << " \rProgress " << std::fixed << (100.0 * (double)(o + k) / (double)n_pri) << "%" << std::flush;
+        }
+        out.flush();
+        std::fflush(stdout);
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::clog << "\nSimulation (persistent, " << launches << " launches): " << n_pri << " primaries, " << out.count
+                  << " detected electrons, " << sec << " s\n";
+        nbl_driver_destroy(d);
+        nbl_materials_destroy(mats);
+        nbl_geometry_destroy(g);
+        return 0;
+    }
     const auto t0 = std::chrono::steady_clock::now();
 
     // prescan (gpu_main.cu:108-166): frame_size = 1 + iteration of the largest

@@ -68,6 +68,14 @@ struct pers_params {
     uint32_t refill_threshold;
     dmlp mlp[2];                        // learned sampler (elastic, inelastic) when LEARNED
     uint32_t learned;                   // bit k: network k replaces table k (LEARNED)
+    // explicit primaries (plist != nullptr): primary i is plist[i] with tag ptags[i]
+    const dparticle* plist;
+    const uint32_t* ptags;
+    // detected-electron records (det == nullptr: off), in detection order
+    dparticle* det;
+    uint32_t* det_tags;
+    unsigned long long* det_count;      // all detections (may exceed det_cap)
+    uint64_t det_cap;
     // energy-deposition tally (dep == nullptr: off)
     double* dep;                        // [dnz][dny][dnx] voxels
     double* dep_total;                  // all deposits
@@ -120,6 +128,11 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
 // Primary i: pixel by binary search, position from a counter-based hash.
 __device__ void make_primary(const pers_params& P, uint64_t i, dparticle& p, uint32_t& tag)
 {
+    if (P.plist) {                                      // explicit primaries
+        p = P.plist[i];
+        tag = P.ptags[i];
+        return;
+    }
     uint32_t lo = 0, hi = P.n_pixels;                    // offsets[lo] <= i < offsets[hi]
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -284,8 +297,8 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
                 const int res = intersect_execute(P.mats, t, p, mat, rng);
                 if (res == OUT_DETECTED) {
                     alive = false;
-                    if (p.e < 50) { se = true; atomicAdd(&P.pixel_se[tag], 1u); }
-                    else { bse = true; atomicAdd(&P.pixel_bse[tag], 1u); }
+                    if (p.e < 50) { se = true; if (P.pixel_se) atomicAdd(&P.pixel_se[tag], 1u); }
+                    else { bse = true; if (P.pixel_bse) atomicAdd(&P.pixel_bse[tag], 1u); }
                     const int bin = (int)p.e;
                     if (bin >= 0 && bin < NBL_HIST_BINS) atomicAdd(&P.hist[bin], 1ull);
                 } else if (res == OUT_TERMINATED) {
@@ -345,6 +358,19 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
         }
         n_se += __popcll(__ballot(se));
         n_bse += __popcll(__ballot(bse));
+        // ---- detection records: one counter update per wavefront -------------
+        if (P.det) {
+            const uint64_t dm = __ballot(se || bse);
+            if (dm) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(P.det_count, (unsigned long long)__popcll(dm));
+                base = bcast64(base);
+                if (se || bse) {
+                    const uint64_t k = base + rank_below(dm);
+                    if (k < P.det_cap) { P.det[k] = p; P.det_tags[k] = tag; }
+                }
+            }
+        }
         // ---- push secondaries ------------------------------------------------
         const uint64_t mm = __ballot(made);
         const uint32_t n_made = __popcll(mm);
@@ -373,6 +399,177 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
         for (int off = 32; off > 0; off >>= 1) dep_sum += __shfl_xor(dep_sum, off);
         if (lane == 0 && dep_sum != 0) atomicAdd(P.dep_total, dep_sum);
     }
+}
+
+// Where the primaries of a persistent run come from: a pixel batch cut into
+// kernel pixels (kp, offsets), or an explicit list of particles with tags.
+struct primary_source {
+    std::vector<dpixel> kp;
+    std::vector<uint64_t> offsets{ 0 };
+    uint32_t n_tally = 0;               // per-pixel tally entries (the caller's pixels)
+    f3 dir{ 0, 0, -1 };
+    uint64_t seed = 0;
+    const nbl_particle* list = nullptr;
+    const uint32_t* tags = nullptr;
+    uint64_t n_list = 0;
+};
+
+void run_persistent(nbl_driver d, const primary_source& src, const nbl_persistent_opts* opts, nbl_tallies* out)
+{
+    const uint64_t total = src.list ? src.n_list : src.offsets.back();
+    const uint32_t nkp = (uint32_t)src.kp.size();
+    const uint32_t np = src.n_tally;
+    require(total > 0, "nbl_driver_run_persistent: no primaries in the batch");
+
+    int dev = 0, cus = 0;
+    NBL_HIP(hipGetDevice(&dev));
+    NBL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    uint32_t block = opts && opts->block_size ? opts->block_size : 256;
+    // the learned variant holds 3 wavefronts per SIMD (12 per CU), the table variant 4
+    uint32_t per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : (d->sampler ? 3u : 4u);
+    uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 128;
+    uint32_t thresh = opts && opts->refill_threshold ? opts->refill_threshold : 1;
+    uint32_t ovf = opts && opts->overflow_depth ? opts->overflow_depth : 4096;
+    require(ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
+    require(block % 64 == 0 && block >= 64 && block <= 1024, "nbl_driver_run_persistent: block_size must be 64..1024, multiple of 64");
+    require(!d->sampler || block <= 768, "nbl_driver_run_persistent: block_size must be <= 768 with a learned sampler");
+    const bool penn = materials_view(d->mats, 0).model == INEL_PENN;
+    require(!(penn && d->sampler), "nbl_driver_run_persistent: the learned sampler covers Kieft inelastic tables only");
+    require(per_cu >= 1 && per_cu <= 32, "nbl_driver_run_persistent: blocks_per_cu must be 1..32");
+    require(depth >= 64 && depth <= 1024, "nbl_driver_run_persistent: stack_depth must be 64..1024");
+    require(thresh >= 1 && thresh <= 64, "nbl_driver_run_persistent: refill_threshold must be 1..64");
+    const size_t lds = (size_t)(block / 64) * (9 * depth + lds_path_cache::kLevels * 64) * sizeof(float);
+    require(lds <= 160 * 1024, "nbl_driver_run_persistent: stack_depth x waves exceeds the LDS size");
+
+    dbuf<dpixel> dpix(std::max<uint32_t>(nkp, 1));
+    dbuf<uint64_t> doff(nkp + 1);
+    dbuf<unsigned long long> dnext(N_QUEUES * QSTRIDE), dhist(NBL_HIST_BINS), dcnt(N_COUNTERS);
+    dbuf<uint32_t> dse(std::max<uint32_t>(np, 1)), dbse(std::max<uint32_t>(np, 1));
+    hipStream_t s = d->stream;
+    if (nkp) {
+        dpix.upload(src.kp.data(), nkp, s);
+        doff.upload(src.offsets.data(), nkp + 1, s);
+    }
+    // explicit primaries and detection records
+    dbuf<dparticle> dlist, ddet;
+    dbuf<uint32_t> dltags, ddtags;
+    dbuf<unsigned long long> ddcnt(1);
+    NBL_HIP(hipMemsetAsync(ddcnt.p, 0, sizeof(unsigned long long), s));
+    if (src.list) {
+        static_assert(sizeof(dparticle) == sizeof(nbl_particle), "particle layout");
+        dlist.alloc(total); dltags.alloc(total);
+        dlist.upload(reinterpret_cast<const dparticle*>(src.list), total, s);
+        dltags.upload(src.tags, total, s);
+    }
+    nbl_detection_buffer* db = out->detected;
+    if (db) {
+        require(db->capacity == 0 || (db->particles && db->tags),
+            "nbl_driver_run_persistent: detection buffer needs particle and tag arrays");
+        if (db->capacity) { ddet.alloc(db->capacity); ddtags.alloc(db->capacity); }
+    }
+    NBL_HIP(hipMemsetAsync(dnext.p, 0, N_QUEUES * QSTRIDE * sizeof(unsigned long long), s));
+    NBL_HIP(hipMemsetAsync(dhist.p, 0, NBL_HIST_BINS * sizeof(unsigned long long), s));
+    NBL_HIP(hipMemsetAsync(dcnt.p, 0, N_COUNTERS * sizeof(unsigned long long), s));
+    NBL_HIP(hipMemsetAsync(dse.p, 0, std::max<uint32_t>(np, 1) * sizeof(uint32_t), s));
+    NBL_HIP(hipMemsetAsync(dbse.p, 0, std::max<uint32_t>(np, 1) * sizeof(uint32_t), s));
+
+    pers_params P;
+    P.g = geometry_view(d->geom);
+    P.mats = materials_device(d->mats);
+    P.min_e = d->min_energy; P.max_e = d->max_energy;
+    P.pixels = dpix.p; P.offsets = doff.p; P.n_pixels = nkp; P.total = total;
+    P.plist = src.list ? dlist.p : nullptr; P.ptags = src.list ? dltags.p : nullptr;
+    P.det = db ? ddet.p : nullptr; P.det_tags = db ? ddtags.p : nullptr;
+    P.det_count = ddcnt.p; P.det_cap = db ? db->capacity : 0;
+    if (db && db->capacity == 0) P.det = reinterpret_cast<dparticle*>(ddcnt.p);   // count only
+    for (uint32_t q = 0; q <= N_QUEUES; ++q) P.qbeg[q] = total * q / N_QUEUES;   // contiguous eighths
+    P.dir = src.dir;
+    P.prim_seed = src.seed;
+    P.rng_seed = d->seed;
+    const uint64_t threads = (uint64_t)cus * per_cu * block;
+    P.rng_first = (uint64_t)d->cap + (d->runs++) * threads;   // streams distinct from the driver's slots
+    P.next = dnext.p; P.hist = dhist.p; P.counters = dcnt.p;
+    P.pixel_se = np ? dse.p : nullptr; P.pixel_bse = np ? dbse.p : nullptr;
+    const size_t n_waves = (size_t)cus * per_cu * (block / 64);
+    const size_t ovf_floats = n_waves * 9 * ovf;
+    if (d->overflow.n < ovf_floats) d->overflow.alloc(ovf_floats);
+    P.overflow = d->overflow.p;
+    P.depth = depth; P.ovf = ovf; P.refill_threshold = thresh;
+    // energy-deposition tally
+    nbl_deposit_grid* dg = out->deposit;
+    dbuf<double> ddep, dtot;
+    P.dep = nullptr; P.dep_total = nullptr;
+    if (dg) {
+        require(dg->energy != nullptr && dg->nx > 0 && dg->ny > 0 && dg->nz > 0,
+            "nbl_driver_run_persistent: deposit grid needs nx, ny, nz > 0 and an energy array");
+        require((uint64_t)dg->nx * dg->ny * dg->nz <= (1ull << 30), "nbl_driver_run_persistent: deposit grid too large");
+        require(dg->hi.x > dg->lo.x && dg->hi.y > dg->lo.y && dg->hi.z > dg->lo.z,
+            "nbl_driver_run_persistent: deposit grid box is empty");
+        const size_t nv = (size_t)dg->nx * dg->ny * dg->nz;
+        ddep.alloc(nv); dtot.alloc(1);
+        NBL_HIP(hipMemsetAsync(ddep.p, 0, nv * sizeof(double), s));
+        NBL_HIP(hipMemsetAsync(dtot.p, 0, sizeof(double), s));
+        P.dep = ddep.p; P.dep_total = dtot.p;
+        P.dnx = dg->nx; P.dny = dg->ny; P.dnz = dg->nz;
+        P.dlo = { dg->lo.x, dg->lo.y, dg->lo.z };
+        P.dinv = { (float)dg->nx / (dg->hi.x - dg->lo.x), (float)dg->ny / (dg->hi.y - dg->lo.y),
+                   (float)dg->nz / (dg->hi.z - dg->lo.z) };
+    }
+
+    hipEvent_t e0, e1;
+    NBL_HIP(hipEventCreate(&e0)); NBL_HIP(hipEventCreate(&e1));
+    NBL_HIP(hipEventRecord(e0, s));
+    const dim3 grid((unsigned)(cus * per_cu)), threads_per_block(block);
+    if (d->sampler) {
+        const dmlp* v = sampler_views(d->sampler);
+        P.mlp[0] = v[0]; P.mlp[1] = v[1];
+        P.learned = sampler_kinds(d->sampler);
+        if (dg) hipLaunchKernelGGL((k_persistent<true, true, false>), grid, threads_per_block, lds, s, P);
+        else hipLaunchKernelGGL((k_persistent<true, false, false>), grid, threads_per_block, lds, s, P);
+    } else if (penn) {
+        if (dg) hipLaunchKernelGGL((k_persistent<false, true, true>), grid, threads_per_block, lds, s, P);
+        else hipLaunchKernelGGL((k_persistent<false, false, true>), grid, threads_per_block, lds, s, P);
+    } else {
+        if (dg) hipLaunchKernelGGL((k_persistent<false, true, false>), grid, threads_per_block, lds, s, P);
+        else hipLaunchKernelGGL((k_persistent<false, false, false>), grid, threads_per_block, lds, s, P);
+    }
+    NBL_HIP(hipGetLastError());
+    NBL_HIP(hipEventRecord(e1, s));
+    NBL_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    NBL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+
+    uint64_t cnt[N_COUNTERS];
+    dcnt.download(reinterpret_cast<unsigned long long*>(cnt), N_COUNTERS, s);
+    std::vector<unsigned long long> hist(NBL_HIST_BINS);
+    dhist.download(hist.data(), NBL_HIST_BINS, s);
+    if (out->pixel_se && np) dse.download(out->pixel_se, np, s);
+    if (out->pixel_bse && np) dbse.download(out->pixel_bse, np, s);
+    unsigned long long n_det = 0;
+    ddcnt.download(&n_det, 1, s);
+    NBL_HIP(hipStreamSynchronize(s));
+    if (db) {
+        db->count = n_det;
+        const uint64_t k = std::min<uint64_t>(n_det, db->capacity);
+        if (k) {
+            ddet.download(reinterpret_cast<dparticle*>(db->particles), k, s);
+            ddtags.download(db->tags, k, s);
+        }
+    }
+    if (dg) {
+        ddep.download(dg->energy, (size_t)dg->nx * dg->ny * dg->nz, s);
+        dtot.download(&dg->total, 1, s);
+    }
+    NBL_HIP(hipStreamSynchronize(s));
+    if (out->hist) for (int b = 0; b < NBL_HIST_BINS; ++b) out->hist[b] = (uint32_t)hist[b];
+    out->primaries = cnt[C_PRIM]; out->detected_se = cnt[C_SE]; out->detected_bse = cnt[C_BSE];
+    out->events = cnt[C_EVENTS]; out->lookups_1d = cnt[C_L1D]; out->lookups_2d = cnt[C_L2D];
+    out->rounddowns = cnt[C_RD]; out->node_steps = cnt[C_NODES]; out->tri_tests = cnt[C_TRIS];
+    out->boundary_events = cnt[C_BOUND]; out->secondaries = cnt[C_SEC]; out->pending_retries = cnt[C_PEND];
+    out->stopped = cnt[C_KILLED];
+    if (dg) dg->count = cnt[C_DEPN];
+    out->kernel_ms = ms;
 }
 
 }  // namespace
@@ -417,127 +614,38 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch, const 
                 offsets.push_back(offsets.back() + px.n_primaries);
             }
         }
-        const uint64_t total = offsets.back();
-        const uint32_t nkp = (uint32_t)kp.size();
-        require(total > 0, "nbl_driver_run_persistent: no primaries in the batch");
         const f3 dir{ batch->dir.x, batch->dir.y, batch->dir.z };
         const float dn = std::sqrt(dir.x * dir.x + dir.y * dir.y + dir.z * dir.z);
         require(dn > 0 && std::isfinite(dn), "nbl_driver_run_persistent: beam direction is zero");
+        primary_source src;
+        src.kp = std::move(kp);
+        src.offsets = std::move(offsets);
+        src.n_tally = np;
+        src.dir = { dir.x / dn, dir.y / dn, dir.z / dn };
+        src.seed = batch->seed;
+        run_persistent(d, src, opts, out);
+    });
+}
 
-        int dev = 0, cus = 0;
-        NBL_HIP(hipGetDevice(&dev));
-        NBL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        uint32_t block = opts && opts->block_size ? opts->block_size : 256;
-        // the learned variant holds 3 wavefronts per SIMD (12 per CU), the table variant 4
-        uint32_t per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : (d->sampler ? 3u : 4u);
-        uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 128;
-        uint32_t thresh = opts && opts->refill_threshold ? opts->refill_threshold : 1;
-        uint32_t ovf = opts && opts->overflow_depth ? opts->overflow_depth : 4096;
-        require(ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
-        require(block % 64 == 0 && block >= 64 && block <= 1024, "nbl_driver_run_persistent: block_size must be 64..1024, multiple of 64");
-        require(!d->sampler || block <= 768, "nbl_driver_run_persistent: block_size must be <= 768 with a learned sampler");
-        const bool penn = materials_view(d->mats, 0).model == INEL_PENN;
-        require(!(penn && d->sampler), "nbl_driver_run_persistent: the learned sampler covers Kieft inelastic tables only");
-        require(per_cu >= 1 && per_cu <= 32, "nbl_driver_run_persistent: blocks_per_cu must be 1..32");
-        require(depth >= 64 && depth <= 1024, "nbl_driver_run_persistent: stack_depth must be 64..1024");
-        require(thresh >= 1 && thresh <= 64, "nbl_driver_run_persistent: refill_threshold must be 1..64");
-        const size_t lds = (size_t)(block / 64) * (9 * depth + lds_path_cache::kLevels * 64) * sizeof(float);
-        require(lds <= 160 * 1024, "nbl_driver_run_persistent: stack_depth x waves exceeds the LDS size");
-
-        dbuf<dpixel> dpix(nkp);
-        dbuf<uint64_t> doff(nkp + 1);
-        dbuf<unsigned long long> dnext(N_QUEUES * QSTRIDE), dhist(NBL_HIST_BINS), dcnt(N_COUNTERS);
-        dbuf<uint32_t> dse(np), dbse(np);
-        hipStream_t s = d->stream;
-        dpix.upload(kp.data(), nkp, s);
-        doff.upload(offsets.data(), nkp + 1, s);
-        NBL_HIP(hipMemsetAsync(dnext.p, 0, N_QUEUES * QSTRIDE * sizeof(unsigned long long), s));
-        NBL_HIP(hipMemsetAsync(dhist.p, 0, NBL_HIST_BINS * sizeof(unsigned long long), s));
-        NBL_HIP(hipMemsetAsync(dcnt.p, 0, N_COUNTERS * sizeof(unsigned long long), s));
-        NBL_HIP(hipMemsetAsync(dse.p, 0, np * sizeof(uint32_t), s));
-        NBL_HIP(hipMemsetAsync(dbse.p, 0, np * sizeof(uint32_t), s));
-
-        pers_params P;
-        P.g = geometry_view(d->geom);
-        P.mats = materials_device(d->mats);
-        P.min_e = d->min_energy; P.max_e = d->max_energy;
-        P.pixels = dpix.p; P.offsets = doff.p; P.n_pixels = nkp; P.total = total;
-        for (uint32_t q = 0; q <= N_QUEUES; ++q) P.qbeg[q] = total * q / N_QUEUES;   // contiguous eighths
-        P.dir = { dir.x / dn, dir.y / dn, dir.z / dn };
-        P.prim_seed = batch->seed;
-        P.rng_seed = d->seed;
-        const uint64_t threads = (uint64_t)cus * per_cu * block;
-        P.rng_first = (uint64_t)d->cap + (d->runs++) * threads;   // streams distinct from the driver's slots
-        P.next = dnext.p; P.pixel_se = dse.p; P.pixel_bse = dbse.p; P.hist = dhist.p; P.counters = dcnt.p;
-        const size_t n_waves = (size_t)cus * per_cu * (block / 64);
-        const size_t ovf_floats = n_waves * 9 * ovf;
-        if (d->overflow.n < ovf_floats) d->overflow.alloc(ovf_floats);
-        P.overflow = d->overflow.p;
-        P.depth = depth; P.ovf = ovf; P.refill_threshold = thresh;
-        // energy-deposition tally
-        nbl_deposit_grid* dg = out->deposit;
-        dbuf<double> ddep, dtot;
-        P.dep = nullptr; P.dep_total = nullptr;
-        if (dg) {
-            require(dg->energy != nullptr && dg->nx > 0 && dg->ny > 0 && dg->nz > 0,
-                "nbl_driver_run_persistent: deposit grid needs nx, ny, nz > 0 and an energy array");
-            require((uint64_t)dg->nx * dg->ny * dg->nz <= (1ull << 30), "nbl_driver_run_persistent: deposit grid too large");
-            require(dg->hi.x > dg->lo.x && dg->hi.y > dg->lo.y && dg->hi.z > dg->lo.z,
-                "nbl_driver_run_persistent: deposit grid box is empty");
-            const size_t nv = (size_t)dg->nx * dg->ny * dg->nz;
-            ddep.alloc(nv); dtot.alloc(1);
-            NBL_HIP(hipMemsetAsync(ddep.p, 0, nv * sizeof(double), s));
-            NBL_HIP(hipMemsetAsync(dtot.p, 0, sizeof(double), s));
-            P.dep = ddep.p; P.dep_total = dtot.p;
-            P.dnx = dg->nx; P.dny = dg->ny; P.dnz = dg->nz;
-            P.dlo = { dg->lo.x, dg->lo.y, dg->lo.z };
-            P.dinv = { (float)dg->nx / (dg->hi.x - dg->lo.x), (float)dg->ny / (dg->hi.y - dg->lo.y),
-                       (float)dg->nz / (dg->hi.z - dg->lo.z) };
+int nbl_driver_run_persistent_list(nbl_driver d, const nbl_particle* primaries, const uint32_t* tags, uint64_t n,
+    const nbl_persistent_opts* opts, nbl_tallies* out)
+{
+    return guarded([&] {
+        require(d != nullptr, "nbl_driver_run_persistent_list: NULL driver");
+        require(out != nullptr, "nbl_driver_run_persistent_list: NULL tallies");
+        require(n > 0 && primaries && tags, "nbl_driver_run_persistent_list: no primaries");
+        for (uint64_t i = 0; i < n; ++i) {
+            const nbl_particle& q = primaries[i];
+            const float dd = q.dir.x * q.dir.x + q.dir.y * q.dir.y + q.dir.z * q.dir.z;
+            require(std::isfinite(q.pos.x) && std::isfinite(q.pos.y) && std::isfinite(q.pos.z) &&
+                    std::isfinite(dd) && dd > 0 && std::isfinite(q.kin_energy) && q.kin_energy > 0,
+                "nbl_driver_run_persistent_list: a primary has a non-finite position, a zero direction or no energy");
         }
-
-        hipEvent_t e0, e1;
-        NBL_HIP(hipEventCreate(&e0)); NBL_HIP(hipEventCreate(&e1));
-        NBL_HIP(hipEventRecord(e0, s));
-        const dim3 grid((unsigned)(cus * per_cu)), threads_per_block(block);
-        if (d->sampler) {
-            const dmlp* v = sampler_views(d->sampler);
-            P.mlp[0] = v[0]; P.mlp[1] = v[1];
-            P.learned = sampler_kinds(d->sampler);
-            if (dg) hipLaunchKernelGGL((k_persistent<true, true, false>), grid, threads_per_block, lds, s, P);
-            else hipLaunchKernelGGL((k_persistent<true, false, false>), grid, threads_per_block, lds, s, P);
-        } else if (penn) {
-            if (dg) hipLaunchKernelGGL((k_persistent<false, true, true>), grid, threads_per_block, lds, s, P);
-            else hipLaunchKernelGGL((k_persistent<false, false, true>), grid, threads_per_block, lds, s, P);
-        } else {
-            if (dg) hipLaunchKernelGGL((k_persistent<false, true, false>), grid, threads_per_block, lds, s, P);
-            else hipLaunchKernelGGL((k_persistent<false, false, false>), grid, threads_per_block, lds, s, P);
-        }
-        NBL_HIP(hipGetLastError());
-        NBL_HIP(hipEventRecord(e1, s));
-        NBL_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        NBL_HIP(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
-
-        uint64_t cnt[N_COUNTERS];
-        dcnt.download(reinterpret_cast<unsigned long long*>(cnt), N_COUNTERS, s);
-        std::vector<unsigned long long> hist(NBL_HIST_BINS);
-        dhist.download(hist.data(), NBL_HIST_BINS, s);
-        if (out->pixel_se) dse.download(out->pixel_se, np, s);
-        if (out->pixel_bse) dbse.download(out->pixel_bse, np, s);
-        if (dg) {
-            ddep.download(dg->energy, (size_t)dg->nx * dg->ny * dg->nz, s);
-            dtot.download(&dg->total, 1, s);
-        }
-        NBL_HIP(hipStreamSynchronize(s));
-        if (out->hist) for (int b = 0; b < NBL_HIST_BINS; ++b) out->hist[b] = (uint32_t)hist[b];
-        out->primaries = cnt[C_PRIM]; out->detected_se = cnt[C_SE]; out->detected_bse = cnt[C_BSE];
-        out->events = cnt[C_EVENTS]; out->lookups_1d = cnt[C_L1D]; out->lookups_2d = cnt[C_L2D];
-        out->rounddowns = cnt[C_RD]; out->node_steps = cnt[C_NODES]; out->tri_tests = cnt[C_TRIS];
-        out->boundary_events = cnt[C_BOUND]; out->secondaries = cnt[C_SEC]; out->pending_retries = cnt[C_PEND];
-        out->stopped = cnt[C_KILLED];
-        if (dg) dg->count = cnt[C_DEPN];
-        out->kernel_ms = ms;
+        require(!out->pixel_se && !out->pixel_bse,
+            "nbl_driver_run_persistent_list: per-pixel tallies need a pixel batch (use the detection records)");
+        primary_source src;
+        src.list = primaries; src.tags = tags; src.n_list = n;
+        run_persistent(d, src, opts, out);
     });
 }
 

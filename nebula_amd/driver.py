@@ -158,19 +158,36 @@ class Driver:
         self.sampler = sampler                 # keep it alive while the driver uses it
 
     def run_persistent(self, beam, block_size=0, blocks_per_cu=0, stack_depth=0, refill_threshold=0,
-                       per_pixel=False, hist=False, deposit=None):
+                       per_pixel=False, hist=False, deposit=None, detected=0):
         """One persistent launch over the pixel batch `beam`.  deposit =
         dict(shape=(nx, ny, nz), lo=(x, y, z), hi=(x, y, z)) turns on the
         energy-deposition tally: out["deposit"] is the [nz, ny, nx] grid (eV),
-        out["deposit_total"] and out["deposit_count"] cover every deposit."""
+        out["deposit_total"] and out["deposit_count"] cover every deposit.
+        detected = capacity of the detected-electron record buffer (0: off):
+        out["detected"] = (particles, tags), out["detected_count"]."""
         px = np.ascontiguousarray(beam["pixels"], L.PIXEL_DTYPE)
         d = beam.get("dir", (0.0, 0.0, -1.0))
         b = L.PixelBatch(pixels=ptr(px), n_pixels=len(px), dir=L.Vec3(*d), seed=int(beam.get("seed", 0)))
+        return self._persistent("nbl_driver_run_persistent", (C.byref(b),), len(px), block_size, blocks_per_cu,
+                                stack_depth, refill_threshold, per_pixel, hist, deposit, detected)
+
+    def run_persistent_list(self, primaries, tags=None, block_size=0, blocks_per_cu=0, stack_depth=0,
+                            refill_threshold=0, hist=False, deposit=None, detected=0):
+        """The persistent transport over explicit primaries (PARTICLE_DTYPE),
+        tags default to their indices; per-pixel tallies are not available."""
+        prim = np.ascontiguousarray(primaries, L.PARTICLE_DTYPE)
+        tags = np.arange(len(prim), dtype=np.uint32) if tags is None else np.ascontiguousarray(tags, np.uint32)
+        self._keep_list = (prim, tags)
+        return self._persistent("nbl_driver_run_persistent_list", (ptr(prim), ptr(tags), len(prim)), 0, block_size,
+                                blocks_per_cu, stack_depth, refill_threshold, False, hist, deposit, detected)
+
+    def _persistent(self, fn, head, n_px, block_size, blocks_per_cu, stack_depth, refill_threshold, per_pixel,
+                    hist, deposit, detected):
         o = L.PersistentOpts(block_size=block_size, blocks_per_cu=blocks_per_cu, stack_depth=stack_depth,
                              refill_threshold=refill_threshold)
         t = L.Tallies()
-        se = np.zeros(len(px), np.uint32) if per_pixel else None
-        bse = np.zeros(len(px), np.uint32) if per_pixel else None
+        se = np.zeros(n_px, np.uint32) if per_pixel else None
+        bse = np.zeros(n_px, np.uint32) if per_pixel else None
         h = np.zeros(L.HIST_BINS, np.uint32) if hist else None
         t.pixel_se, t.pixel_bse, t.hist = ptr(se), ptr(bse), ptr(h)
         dep = grid = None
@@ -180,9 +197,18 @@ class Driver:
             dep = L.DepositGrid(nx=nx, ny=ny, nz=nz, lo=L.Vec3(*map(float, deposit["lo"])),
                                 hi=L.Vec3(*map(float, deposit["hi"])), energy=ptr(grid))
             t.deposit = C.cast(C.pointer(dep), C.c_void_p)
-        call("nbl_driver_run_persistent", self.h, C.byref(b), C.byref(o), C.byref(t))
+        det = dparts = dtags = None
+        if detected:
+            dparts = np.zeros(int(detected), L.PARTICLE_DTYPE)
+            dtags = np.zeros(int(detected), np.uint32)
+            det = L.DetectionBuffer(particles=ptr(dparts), tags=ptr(dtags), capacity=int(detected))
+            t.detected = C.cast(C.pointer(det), C.c_void_p)
+        call(fn, self.h, *head, C.byref(o), C.byref(t))
         out = {f: getattr(t, f) for f, _ in L.Tallies._fields_
-               if f not in ("pixel_se", "pixel_bse", "hist", "deposit")}
+               if f not in ("pixel_se", "pixel_bse", "hist", "deposit", "detected")}
+        if det is not None:
+            k = min(det.count, int(detected))
+            out["detected"], out["detected_count"] = (dparts[:k], dtags[:k]), det.count
         if dep is not None:
             out["deposit"], out["deposit_total"], out["deposit_count"] = grid, dep.total, dep.count
         if per_pixel:

@@ -115,3 +115,33 @@ def test_cli_reads_hdf5_materials(tmp_path):
     r = subprocess.run([CLI, "--inelastic", "penn", *map(str, common), str(tmp_path / "si.nblmat")],
                        capture_output=True, timeout=120)
     assert r.returncode == 1 and b"Full Penn model not found" in r.stderr
+
+
+def test_cli_persistent_mode(tmp_path):
+    """--persistent 1: the same files through the persistent kernel; the
+    records sit on the detector with valid pixels and the yields agree with
+    the frame loop's (same seed, different random streams) within 4 sigma."""
+    print("[gpu] nbl_gpu --persistent", flush=True)
+    tris = scenes.cube_slab()
+    a = tris["r0"].astype(np.float64)
+    files.write_tri_file(tmp_path / "g.tri", tris["material_in"], tris["material_out"], a, a + tris["e1"],
+                         a + tris["e2"])
+    n = 50_000
+    prim = runs.pencil_primaries(n, 1000.0)
+    pix = np.stack([np.arange(n) % 100, np.arange(n) // 100], axis=1).astype(np.int32)
+    files.write_pri_file(tmp_path / "p.pri", prim, pix)
+    synth.write_material_file(synth.SI, tmp_path / "si.nblmat")
+    common = ["--capacity", "200000", "--seed", "7", tmp_path / "g.tri", tmp_path / "p.pri", tmp_path / "si.nblmat"]
+    out_p, err = _cli("--persistent", "1", *common)
+    assert "persistent" in err
+    out_f, _ = _cli(*common)
+    rp, rf = files.read_records(out_p), files.read_records(out_f)
+    assert len(rp) > 0 and np.all(np.abs(rp["pos"][:, 2] - 10.0) < 1e-3)
+    assert np.all((rp["pixel"][:, 0] >= 0) & (rp["pixel"][:, 0] < 100) & (rp["pixel"][:, 1] >= 0) &
+                  (rp["pixel"][:, 1] < n // 100))
+    for lo, hi in ((0, 50), (50, 1e9)):
+        kp = int(((rp["kin_energy"] >= lo) & (rp["kin_energy"] < hi)).sum())
+        kf = int(((rf["kin_energy"] >= lo) & (rf["kin_energy"] < hi)).sum())
+        z = runs.yield_z(kp, n, kf, n)
+        print(f"  [{lo}, {hi}) eV: persistent {kp / n:.4f} frame loop {kf / n:.4f} z {z:+.2f}")
+        assert abs(z) < 4

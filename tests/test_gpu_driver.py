@@ -198,3 +198,39 @@ def test_iteration_driver_and_persistent_agree():
         z = runs.yield_z(k_it, n, k_p, t["primaries"])
         print(f"  iteration {k_it / n:.4f} persistent {k_p / t['primaries']:.4f} z {z:+.2f}")
         assert abs(z) < 4
+
+
+def test_persistent_list_mode_and_detection_records():
+    """Explicit primaries through the persistent kernel, detected electrons
+    returned as records: the records agree with the tallies of the same
+    launch (count, SE/BSE split, 1-eV histogram, detector plane, tags), the
+    yields with the oracle's within 3 sigma; a too-small buffer keeps the
+    count; bad primaries and per-pixel tallies are refused."""
+    print("[gpu] persistent list mode + detection records", flush=True)
+    tris = scenes.cube_slab()
+    g = na.Geometry(tris); m = na.Materials([synth.SI])
+    og = ol.OrcGeometry(tris); om = ol.OrcMaterials([m.prepared[0].desc])
+    st, _ = runs.oracle_run(og, om, runs.pencil_primaries(12_000, 1000.0), threads=16)
+    n = 200_000
+    prim = runs.pencil_primaries(n, 1000.0)
+    tags = (np.arange(n) % 1000).astype(np.uint32)
+    with Driver(g, m, capacity=1 << 16, seed=17) as drv:
+        t = drv.run_persistent_list(prim, tags, hist=True, detected=4 * n)
+        parts, dtags = t["detected"]
+        assert t["primaries"] == n and t["stopped"] == 0
+        assert t["detected_count"] == len(parts) == t["detected_se"] + t["detected_bse"]
+        e = parts["kin_energy"]
+        assert (e < 50).sum() == t["detected_se"] and (e >= 50).sum() == t["detected_bse"]
+        h = np.bincount(np.floor(e[e < L.HIST_BINS]).astype(np.int64), minlength=L.HIST_BINS)
+        assert np.array_equal(h, t["hist"].astype(np.int64))
+        assert np.all(dtags < 1000) and np.all(np.abs(parts["pos"][:, 2] - 10.0) < 1e-3)
+        for key in ("detected_se", "detected_bse"):
+            z = runs.yield_z(t[key], n, st[key], st["primaries"])
+            print(f"  {key}: list {t[key] / n:.4f} oracle {st[key] / st['primaries']:.4f} z {z:+.2f}")
+            assert abs(z) < 3.5
+        small = drv.run_persistent_list(prim[:20_000], detected=100)
+        assert small["detected_count"] > 100 and len(small["detected"][0]) == 100
+        bad = prim[:10].copy()
+        bad["dir"][3] = 0
+        with pytest.raises(na.NblError):
+            drv.run_persistent_list(bad)
