@@ -146,6 +146,29 @@ public:
     void do_iteration() { check(nbl_driver_do_iteration(h_)); }
     nbl_driver handle() const { return h_; }
 
+    // Every primary run to the end of its cascade in persistent launches
+    // (no reference counterpart: replaces the prescan + frame loop of
+    // gpu_main.cu:108-213); function(particle, tag) is called for each
+    // detected electron, in detection order.
+    template<typename detect_function>
+    nbl_tallies run_to_end(const particle* primaries, const uint32_t* tags, uint64_t n, detect_function function,
+        const nbl_persistent_opts* opts = nullptr)
+    {
+        uint64_t cap = 4 * n + 64;
+        for (;;) {
+            std::vector<particle> det(cap);
+            std::vector<uint32_t> det_tags(cap);
+            nbl_detection_buffer db{ det.data(), det_tags.data(), cap, 0 };
+            nbl_tallies t{};
+            t.detected = &db;
+            check(nbl_driver_run_persistent_list(h_, primaries, tags, n, opts, &t));
+            if (db.count > cap) { cap = 2 * db.count; continue; }
+            for (uint64_t i = 0; i < db.count; ++i) function(det[i], det_tags[i]);
+            t.detected = nullptr;
+            return t;
+        }
+    }
+
     float _min_energy, _max_energy;
 
 private:

@@ -97,3 +97,58 @@ def test_kieft_prepare_semantics():
     eta = synth.screening(spec, k)[:, None]
     u = synth.prob_grid()[None, :]
     assert np.allclose(raw.el_cos, 1 - 2 * eta * u / (1 + eta - u), atol=2e-6)
+
+
+FACADE_PROGRAM = r"""
+#include <cstdio>
+#include <stdexcept>
+#include "nbl_hip.hpp"
+// instantiates the facade's templates (not called: it needs a device)
+uint64_t use_templates(nbl_hip::gpu_driver& d, nbl_particle* p, uint32_t* t)
+{
+    uint64_t n = 0;
+    d.run_to_end(p, t, 1, [&](const nbl_particle&, uint32_t) { ++n; });
+    d.flush_detected([&](const nbl_particle&, uint32_t) { ++n; });
+    return n + d.get_running_count();
+}
+int main()
+{
+    int a = 0, b = 0, c = 0;
+    nbl_version(&a, &b, &c);
+    std::printf("version %d.%d.%d\n", a, b, c);
+    try {
+        nbl_hip::octree g(std::vector<nbl_hip::triangle>{});
+        std::printf("no exception\n");
+    } catch (const std::runtime_error& e) {
+        std::printf("runtime_error: %s\n", e.what());
+    }
+    // the persistent entry point's argument checks come before any device work
+    nbl_tallies t{};
+    std::printf("list rc %d\n", nbl_driver_run_persistent_list(nullptr, nullptr, nullptr, 0, nullptr, &t));
+    return 0;
+}
+"""
+
+
+def test_cpp_facade_compiles_and_maps_errors(tmp_path):
+    """include/nbl_hip.hpp compiles against the C ABI with a plain host
+    compiler, links libnbl_hip.so, and turns error codes into the
+    reference's exception types (no device needed for these calls)."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if cxx is None:
+        pytest.skip("no host C++ compiler")
+    src = tmp_path / "facade.cpp"
+    src.write_text(FACADE_PROGRAM)
+    exe = tmp_path / "facade"
+    libdir = os.path.dirname(L.LIB_PATH)
+    r = subprocess.run([cxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-o", str(exe), "-L", libdir, "-lnbl_hip", f"-Wl,-rpath,{libdir}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "version 1.0.2" in r.stdout
+    assert "runtime_error: nbl_geometry_create: no triangles" in r.stdout
+    assert f"list rc {L.NBL_ERR_INVALID}" in r.stdout
