@@ -71,7 +71,8 @@ struct pers_params {
     // explicit primaries (plist != nullptr): primary i is plist[i] with tag ptags[i]
     const dparticle* plist;
     const uint32_t* ptags;
-    // detected-electron records (det == nullptr: off), in detection order
+    // detected-electron records (det_on == 0: off), in detection order
+    uint32_t det_on;
     dparticle* det;
     uint32_t* det_tags;
     unsigned long long* det_count;      // all detections (may exceed det_cap)
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
         n_se += __popcll(__ballot(se));
         n_bse += __popcll(__ballot(bse));
         // ---- detection records: one counter update per wavefront -------------
-        if (P.det) {
+        if (P.det_on) {
             const uint64_t dm = __ballot(se || bse);
             if (dm) {
                 unsigned long long base = 0;
@@ -479,9 +480,9 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
     P.min_e = d->min_energy; P.max_e = d->max_energy;
     P.pixels = dpix.p; P.offsets = doff.p; P.n_pixels = nkp; P.total = total;
     P.plist = src.list ? dlist.p : nullptr; P.ptags = src.list ? dltags.p : nullptr;
-    P.det = db ? ddet.p : nullptr; P.det_tags = db ? ddtags.p : nullptr;
+    P.det_on = db ? 1u : 0u;             // capacity 0: count only
+    P.det = ddet.p; P.det_tags = ddtags.p;
     P.det_count = ddcnt.p; P.det_cap = db ? db->capacity : 0;
-    if (db && db->capacity == 0) P.det = reinterpret_cast<dparticle*>(ddcnt.p);   // count only
     for (uint32_t q = 0; q <= N_QUEUES; ++q) P.qbeg[q] = total * q / N_QUEUES;   // contiguous eighths
     P.dir = src.dir;
     P.prim_seed = src.seed;
