@@ -494,29 +494,41 @@ struct rng_inject {
 template<typename R> __device__ __forceinline__ float rng_phi(R& r) { return 2 * PI_F * r.unit(); }
 template<typename R> __device__ __forceinline__ float rng_exponential(R& r, float tau) { return -tau * logf(r.unit()); }
 
-// make_normal_vec, vector_math.h:171-186
-__device__ __forceinline__ f3 make_normal_vec(f3 dir, float phi)
+// make_normal_vec (vector_math.h:171-186) for the azimuth phi = 2 pi u
+// (random_generator::phi, random.h:64-67), without trigonometric range
+// reduction: sin and cos of atan2(y, x) are y/r and x/r with r = |(x, y)|
+// (0 and 1 for r = 0, as atan2(0, 0) = 0), and sin, cos of 2 pi u come from
+// sincospi(2u).  The same vector as the reference's up to rounding.
+__device__ __forceinline__ f3 make_normal_vec_u(f3 dir, float u)
 {
-    const float az = atan2f(dir.y, dir.x);
-    const float sin_az = sinf(az), cos_az = cosf(az);
-    const f3 unit_v{ dir.z * cos_az, dir.z * sin_az, -sqrtf(dir.x * dir.x + dir.y * dir.y) };
+    const float r = sqrtf(dir.x * dir.x + dir.y * dir.y);
+    float cos_az = 1, sin_az = 0;
+    if (r > 0) {
+        const float ir = 1 / r;
+        cos_az = dir.x * ir;
+        sin_az = dir.y * ir;
+    }
+    const f3 unit_v{ dir.z * cos_az, dir.z * sin_az, -r };
     const f3 unit_u = cross(unit_v, dir);
-    const float sin_phi = sinf(phi), cos_phi = cosf(phi);
+    float sin_phi, cos_phi;
+    sincospif(2 * u, &sin_phi, &cos_phi);
     return unit_u * cos_phi + unit_v * sin_phi;
 }
-// make_unit_vec, vector_math.h:194-206
-__device__ __forceinline__ f3 make_unit_vec(float cos_theta, float phi)
+// make_unit_vec (vector_math.h:194-206) for phi = 2 pi u.
+__device__ __forceinline__ f3 make_unit_vec_u(float cos_theta, float u)
 {
-    const float sin_phi = sinf(phi), cos_phi = cosf(phi);
+    float sin_phi, cos_phi;
+    sincospif(2 * u, &sin_phi, &cos_phi);
     cos_theta = clampr(cos_theta, -1, 1);
     const float sin_theta = sqrtf(1 - cos_theta * cos_theta);
     return { sin_theta * cos_phi, sin_theta * sin_phi, cos_theta };
 }
+// random_generator::uniform_vector (random.h:72-75): cos(theta) first, then phi.
 template<typename R> __device__ __forceinline__ f3 rng_uniform_vector(R& r)
 {
     const float c = 2 * r.unit() - 1;
-    const float p = rng_phi(r);
-    return make_unit_vec(c, p);
+    const float u = r.unit();
+    return make_unit_vec_u(c, u);
 }
 
 struct dparticle { f3 pos, dir; float e; };
@@ -619,7 +631,7 @@ template<typename R>
 __device__ __forceinline__ f3 scatter_frame(dparticle& p, R& rng)
 {
     p.dir = normalised(p.dir);
-    return normalised(make_normal_vec(p.dir, rng_phi(rng)));
+    return normalised(make_normal_vec_u(p.dir, rng.unit()));
 }
 
 // elastic.h:88-115 (phonon + recoil loss on); inelastic.h:151-206.
@@ -714,13 +726,13 @@ __device__ __forceinline__ bool penn_execute(const dmat& m, dparticle& p, R& rng
             const float z = rng.unit();
             q = tab3_get(m.pq, x, y, z);
         }
-        const f3 prim_normal_dir = normalised(make_normal_vec(p.dir, rng_phi(rng)));
+        const f3 prim_normal_dir = normalised(make_normal_vec_u(p.dir, rng.unit()));
         f3 q_dir, q_normal_dir;
         {
             const float c = clampr((omega + q * q) / (2 * sqrtf(p.e) * q), -1, 1);
             const float s = sqrtf(1 - c * c);
             q_dir = normalised(p.dir * c + prim_normal_dir * s);
-            q_normal_dir = normalised(make_normal_vec(q_dir, rng_phi(rng)));
+            q_normal_dir = normalised(make_normal_vec_u(q_dir, rng.unit()));
         }
         {
             const float c = clampr((2 * p.e - omega - q * q) / (2 * sqrtf(p.e * (p.e - omega))), -1, 1);
@@ -769,7 +781,7 @@ __device__ __forceinline__ bool penn_execute(const dmat& m, dparticle& p, R& rng
         return true;
     }
     // inner-shell excitation (as Kieft, inelastic.h:151-206)
-    const f3 normal_dir = normalised(make_normal_vec(p.dir, rng_phi(rng)));
+    const f3 normal_dir = normalised(make_normal_vec_u(p.dir, rng.unit()));
     const float K = p.e - m.fermi + 2 * binding;
     const float dK = binding + omega;
     const float cos_theta = sqrtf(dK / K);
