@@ -494,6 +494,13 @@ struct rng_inject {
 template<typename R> __device__ __forceinline__ float rng_phi(R& r) { return 2 * PI_F * r.unit(); }
 template<typename R> __device__ __forceinline__ float rng_exponential(R& r, float tau) { return -tau * logf(r.unit()); }
 
+// Square root and normalisation in the scattering physics: one hardware
+// instruction each (v_sqrt_f32, v_rsq_f32, about 1 ulp), as the reference's
+// GPU build computes them (-prec-sqrt=false, rnorm3df; vector_math.h:110-158).
+// The traversal and the boundary crossing keep correctly rounded ones.
+__device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ f3 normalised_fast(f3 a) { return a * __builtin_amdgcn_rsqf(dot(a, a)); }
+
 // make_normal_vec (vector_math.h:171-186) for the azimuth phi = 2 pi u
 // (random_generator::phi, random.h:64-67), without trigonometric range
 // reduction: sin and cos of atan2(y, x) are y/r and x/r with r = |(x, y)|
@@ -501,10 +508,10 @@ template<typename R> __device__ __forceinline__ float rng_exponential(R& r, floa
 // sincospi(2u).  The same vector as the reference's up to rounding.
 __device__ __forceinline__ f3 make_normal_vec_u(f3 dir, float u)
 {
-    const float r = sqrtf(dir.x * dir.x + dir.y * dir.y);
+    const float r = sqrt_fast(dir.x * dir.x + dir.y * dir.y);
     float cos_az = 1, sin_az = 0;
     if (r > 0) {
-        const float ir = 1 / r;
+        const float ir = __builtin_amdgcn_rcpf(r);
         cos_az = dir.x * ir;
         sin_az = dir.y * ir;
     }
@@ -520,7 +527,7 @@ __device__ __forceinline__ f3 make_unit_vec_u(float cos_theta, float u)
     float sin_phi, cos_phi;
     sincospif(2 * u, &sin_phi, &cos_phi);
     cos_theta = clampr(cos_theta, -1, 1);
-    const float sin_theta = sqrtf(1 - cos_theta * cos_theta);
+    const float sin_theta = sqrt_fast(1 - cos_theta * cos_theta);
     return { sin_theta * cos_phi, sin_theta * sin_phi, cos_theta };
 }
 // random_generator::uniform_vector (random.h:72-75): cos(theta) first, then phi.
@@ -630,8 +637,8 @@ __device__ __forceinline__ bool scatter_begin(const dmat& m, dparticle& p, R& rn
 template<typename R>
 __device__ __forceinline__ f3 scatter_frame(dparticle& p, R& rng)
 {
-    p.dir = normalised(p.dir);
-    return normalised(make_normal_vec_u(p.dir, rng.unit()));
+    p.dir = normalised_fast(p.dir);
+    return normalised_fast(make_normal_vec_u(p.dir, rng.unit()));
 }
 
 // elastic.h:88-115 (phonon + recoil loss on); inelastic.h:151-206.
@@ -642,7 +649,7 @@ __device__ __forceinline__ bool scatter_end(const dmat& m, dparticle& p, R& rng,
 {
     if (s.elastic) {
         const float cos_theta = s.cos_theta;
-        const float sin_theta = sqrtf(1 - cos_theta * cos_theta);
+        const float sin_theta = sqrt_fast(1 - cos_theta * cos_theta);
         p.dir = p.dir * cos_theta + nd * sin_theta;
         if (p.e < 200) p.e -= minr(50e-3f, m.phonon_loss);
         else p.e -= m.recoil_const * (1 - cos_theta) * p.e;
@@ -651,13 +658,13 @@ __device__ __forceinline__ bool scatter_end(const dmat& m, dparticle& p, R& rng,
     const float binding = s.binding, omega = s.omega;
     const float K = p.e - m.fermi + 2 * binding;
     const float dK = binding + omega;
-    const float cos_theta = sqrtf(dK / K);
-    const float sin_theta = sqrtf(1 - clampr(cos_theta * cos_theta, 0, 1));
+    const float cos_theta = sqrt_fast(dK / K);
+    const float sin_theta = sqrt_fast(1 - clampr(cos_theta * cos_theta, 0, 1));
     f3 sdir = p.dir * cos_theta + nd * sin_theta;
-    sdir = normalised(sdir);
-    const float sb = sqrtf(binding / dK);
+    sdir = normalised_fast(sdir);
+    const float sb = sqrt_fast(binding / dK);
     sdir = sdir + sb * rng_uniform_vector(rng);
-    sdir = normalised(sdir);
+    sdir = normalised_fast(sdir);
     sec->e = m.fermi + omega - binding;
     sec->pos = p.pos;
     sec->dir = sdir;
@@ -712,7 +719,7 @@ __device__ __forceinline__ bool penn_execute(const dmat& m, dparticle& p, R& rng
         const float z = rng.unit();
         binding = tab3_rounddown(m.pion, x, y, z);
     }
-    p.dir = normalised(p.dir);
+    p.dir = normalised_fast(p.dir);
     if (binding <= 0) {
         if (!(omega > m.band_gap)) {        // sub-band-gap loss (optical phonon)
             p.e -= omega;
@@ -726,22 +733,23 @@ __device__ __forceinline__ bool penn_execute(const dmat& m, dparticle& p, R& rng
             const float z = rng.unit();
             q = tab3_get(m.pq, x, y, z);
         }
-        const f3 prim_normal_dir = normalised(make_normal_vec_u(p.dir, rng.unit()));
+        const f3 prim_normal_dir = normalised_fast(make_normal_vec_u(p.dir, rng.unit()));
         f3 q_dir, q_normal_dir;
         {
+            // correctly rounded: c near 1 amplifies its error in sqrt(1 - c^2)
             const float c = clampr((omega + q * q) / (2 * sqrtf(p.e) * q), -1, 1);
-            const float s = sqrtf(1 - c * c);
-            q_dir = normalised(p.dir * c + prim_normal_dir * s);
-            q_normal_dir = normalised(make_normal_vec_u(q_dir, rng.unit()));
+            const float s = sqrt_fast(1 - c * c);
+            q_dir = normalised_fast(p.dir * c + prim_normal_dir * s);
+            q_normal_dir = normalised_fast(make_normal_vec_u(q_dir, rng.unit()));
         }
         {
             const float c = clampr((2 * p.e - omega - q * q) / (2 * sqrtf(p.e * (p.e - omega))), -1, 1);
-            const float s = sqrtf(1 - c * c);
+            const float s = sqrt_fast(1 - c * c);
             p.e -= omega;
             p.dir = p.dir * c + prim_normal_dir * s;
         }
-        const float k_fermi = sqrtf(m.fermi);
-        const float k_fermi_omega = sqrtf(m.fermi + omega);
+        const float k_fermi = sqrt_fast(m.fermi);
+        const float k_fermi_omega = sqrt_fast(m.fermi + omega);
         if (q < k_fermi_omega - k_fermi) {
             // plasmon decay: x = E_i / omega with p(x) ~ sqrt(x (x + 1)) on [max(0, f-1), f]
             const float f = m.fermi / omega;
@@ -771,7 +779,7 @@ __device__ __forceinline__ bool penn_execute(const dmat& m, dparticle& p, R& rng
             const float ki_perp_min_sq = maxr(0, m.fermi - kf_parr * kf_parr);
             const float ki_perp_max_sq = m.fermi - ki_parr * ki_parr;
             const float ki_perp_sq = maxr(0, ki_perp_min_sq + (ki_perp_max_sq - ki_perp_min_sq) * rng.unit());
-            const f3 ki = ki_parr * q_dir + sqrtf(ki_perp_sq) * q_normal_dir;
+            const f3 ki = ki_parr * q_dir + sqrt_fast(ki_perp_sq) * q_normal_dir;
             sec->e = dot(ki, ki) + omega;
             sec->dir = ki + q * q_dir;
         } else {
@@ -781,15 +789,15 @@ __device__ __forceinline__ bool penn_execute(const dmat& m, dparticle& p, R& rng
         return true;
     }
     // inner-shell excitation (as Kieft, inelastic.h:151-206)
-    const f3 normal_dir = normalised(make_normal_vec_u(p.dir, rng.unit()));
+    const f3 normal_dir = normalised_fast(make_normal_vec_u(p.dir, rng.unit()));
     const float K = p.e - m.fermi + 2 * binding;
     const float dK = binding + omega;
-    const float cos_theta = sqrtf(dK / K);
-    const float sin_theta = sqrtf(1 - clampr(cos_theta * cos_theta, 0, 1));
+    const float cos_theta = sqrt_fast(dK / K);
+    const float sin_theta = sqrt_fast(1 - clampr(cos_theta * cos_theta, 0, 1));
     f3 sdir = p.dir * cos_theta + normal_dir * sin_theta;
-    sdir = normalised(sdir);
-    sdir = sdir + sqrtf(binding / dK) * rng_uniform_vector(rng);
-    sdir = normalised(sdir);
+    sdir = normalised_fast(sdir);
+    sdir = sdir + sqrt_fast(binding / dK) * rng_uniform_vector(rng);
+    sdir = normalised_fast(sdir);
     sec->e = m.fermi + omega - binding;
     sec->pos = p.pos;
     sec->dir = sdir;
