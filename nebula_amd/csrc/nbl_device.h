@@ -76,14 +76,18 @@ NBL_HD bool in_domain(const dgeom& g, f3 p)   // octree.inl:24-30
 }
 
 // triangle::intersect_ray, core/triangle.inl:7-28.
-// Most candidates fail the u or v range test.  Those whose barycentrics,
-// estimated with the hardware reciprocal (relative error < 2^-21), lie
-// outside the ranges by a relative margin of 2^-12 are rejected before the
-// correctly rounded divisions: for them the rounded u = fl(u_num / det) of the
-// reference is certainly outside too (the margin exceeds the estimate's error
-// by a factor of 500, and the thresholds' spacing by far more), so the result
-// stays the reference's, bit for bit.  Every other candidate takes the
-// reference's divisions and tests unchanged.
+// The range tests on the barycentrics u = fl(u_num / det), v = fl(v_num / det)
+// are decided first on estimates with the hardware reciprocal (relative
+// error < 2^-21), against the thresholds widened or narrowed by a relative
+// margin of 2^-12 of the estimate:
+//   * clearly outside (most candidates): rejected;
+//   * clearly inside: accepted without the divisions for u and v;
+//   * otherwise, NaN included: the reference's divisions and tests.
+// The margin exceeds the estimate's error (against the exact quotient, and so
+// against the rounded u and v and their rounded sum) by a factor of 250 and
+// the rounding of the shifted thresholds by far more, so every decision is
+// the reference's, and the returned distance is its correctly rounded
+// quotient: results stay bit-identical.
 __device__ __forceinline__ float intersect_ray(f3 r0, f3 e1, f3 e2, f3 start, f3 dir)
 {
     const f3 pvec = cross(dir, e2);
@@ -93,6 +97,7 @@ __device__ __forceinline__ float intersect_ray(f3 r0, f3 e1, f3 e2, f3 start, f3
     const float un = dot(tvec, pvec);
     const f3 qvec = cross(tvec, e1);
     const float vn = dot(dir, qvec);
+    bool inside;
     {
         const float r = __builtin_amdgcn_rcpf(det);
         const float ua = un * r, va = vn * r;
@@ -100,11 +105,15 @@ __device__ __forceinline__ float intersect_ray(f3 r0, f3 e1, f3 e2, f3 start, f3
         if ((ua < -EPSILON - mu) || (ua > (1 + EPSILON) + mu) || (va < -EPSILON - mv) ||
             (ua + va > (1 + EPSILON) + (mu + mv)))
             return -1;
+        inside = (ua > -EPSILON + mu) && (ua < (1 + EPSILON) - mu) && (va > -EPSILON + mv) &&
+                 (ua + va < (1 + EPSILON) - (mu + mv));
     }
-    const float u = un / det;
-    if ((u < -EPSILON) || (u > 1 + EPSILON)) return -1;
-    const float v = vn / det;
-    if ((v < -EPSILON) || (u + v > 1 + EPSILON)) return -1;
+    if (!inside) {
+        const float u = un / det;
+        if ((u < -EPSILON) || (u > 1 + EPSILON)) return -1;
+        const float v = vn / det;
+        if ((v < -EPSILON) || (u + v > 1 + EPSILON)) return -1;
+    }
     const float t = dot(e2, qvec) / det;
     if (t < 0) return -1;
     return t;

@@ -74,6 +74,49 @@ def test_propagate_matches_oracle_full_scenes(scene):
     assert (gt != L.NO_TRIANGLE).mean() > 0.05
 
 
+def test_propagate_rays_at_triangle_edges_and_vertices():
+    """Rays aimed at barycentric points on and around the range thresholds of
+    intersect_ray (u or v = -EPSILON, 0, 1, 1 + EPSILON, u + v = 1): the
+    decisions that the device takes from reciprocal estimates must be the
+    reference's for every one of them, with the distance bit for bit."""
+    print("[gpu] propagate edge and vertex rays", flush=True)
+    rng = np.random.default_rng(5)
+    nt = 300
+    tris = np.zeros(nt, L.TRIANGLE_DTYPE)
+    tris["material_in"], tris["material_out"] = 0, L.MAT_VACUUM
+    tris["r0"] = rng.uniform(-15, 15, (nt, 3))
+    tris["e1"] = rng.standard_normal((nt, 3)) * 3
+    tris["e2"] = rng.standard_normal((nt, 3)) * 3
+    eps = np.float32(1.1920928955078125e-06)
+    edge = [0.0, eps, -eps, 0.5 * eps, -0.5 * eps, 2 * eps, -2 * eps]
+    ab = [(a, b) for a in edge for b in (0.0, 0.5, 1.0 - a)] + \
+         [(1 + d, 0.0) for d in edge] + [(0.0, 1 + d) for d in edge] + \
+         [(0.5 + 0.5 * d, 0.5 + 0.5 * d) for d in edge] + [(0.25 + d, 0.75) for d in edge] + [(1.0, 1.0), (0.3, 0.3)]
+    ab = np.array(ab, np.float64)
+    k = np.repeat(np.arange(nt), len(ab))
+    a = np.tile(ab[:, 0], nt)[:, None]; b = np.tile(ab[:, 1], nt)[:, None]
+    t = tris[k]
+    target = (t["r0"].astype(np.float64) + a * t["e1"] + b * t["e2"]).astype(np.float32)
+    d = rng.standard_normal((len(k), 3)); d = (d / np.linalg.norm(d, axis=1)[:, None]).astype(np.float32)
+    start = (target - 4.0 * d).astype(np.float32)
+    dist = np.full(len(k), 12.0, np.float32)
+    it = np.full(len(k), L.NO_TRIANGLE, np.uint32)
+    im = np.full(len(k), 7, np.int32)                    # no triangle's material: nothing skipped
+    g = na.Geometry(tris)
+    og = ol.OrcGeometry(tris)
+    gd, gt = g.propagate(start, d, dist, it, im)
+    od, ot = og.propagate(start, d, dist, it, im)
+    assert np.array_equal(gt, ot)
+    assert ulp_diff(gd, od).max() == 0
+    aimed = gt == k
+    assert aimed.mean() > 0.3, aimed.mean()              # most rays end on the triangle they were aimed at
+    # both outcomes occur right at the thresholds: some rays aimed just
+    # outside a triangle still hit it (within EPSILON), others miss it
+    outside = (ab[:, 0] < 0) | (ab[:, 1] < 0) | (ab.sum(1) > 1) | (ab[:, 0] > 1) | (ab[:, 1] > 1)
+    o = np.tile(outside, nt)
+    assert 0 < aimed[o].mean() < 1
+
+
 def test_tables_match_reference_fixture_and_oracle():
     print("[gpu] tables", flush=True)
     f = np.load(os.path.join(GOLDEN, "tables.npz"), allow_pickle=False)
