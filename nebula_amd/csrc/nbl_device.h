@@ -8,7 +8,9 @@
 // reference, so that traversal and table interpolation are bit-identical to
 // the reference's CPU code.  Transcendentals (log, exp, expm1, sin, cos,
 // atan2) come from the ROCm device library and may differ from the host libm
-// by an ulp or two.
+// by an ulp or two; parts of the physics use the hardware square root,
+// reciprocal and reciprocal square root (sqrt_fast, rcp_fast,
+// normalised_fast below), about 1 ulp each.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -503,11 +505,14 @@ struct rng_inject {
 template<typename R> __device__ __forceinline__ float rng_phi(R& r) { return 2 * PI_F * r.unit(); }
 template<typename R> __device__ __forceinline__ float rng_exponential(R& r, float tau) { return -tau * logf(r.unit()); }
 
-// Square root and normalisation in the scattering physics: one hardware
-// instruction each (v_sqrt_f32, v_rsq_f32, about 1 ulp), as the reference's
-// GPU build computes them (-prec-sqrt=false, rnorm3df; vector_math.h:110-158).
-// The traversal and the boundary crossing keep correctly rounded ones.
+// Square root and normalisation in the physics: one hardware instruction
+// each (v_sqrt_f32, v_rsq_f32, about 1 ulp), as the reference's GPU build
+// computes them (-prec-sqrt=false, rnorm3df; vector_math.h:110-158).  The
+// same for reciprocals whose error no later step amplifies (v_rcp_f32,
+// about 1 ulp).  The traversal, the table interpolation and the boundary
+// crossing keep correctly rounded operations.
 __device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ f3 normalised_fast(f3 a) { return a * __builtin_amdgcn_rsqf(dot(a, a)); }
 
 // make_normal_vec (vector_math.h:171-186) for the azimuth phi = 2 pi u
@@ -559,12 +564,12 @@ __device__ __forceinline__ float sample_path(const dmat& m, float kin_energy, R&
     uint8_t ty = 0;
     {
         const float imfp = expf(tab1_get(m.el_imfp, le));
-        const float path = rng_exponential(rng, 1 / imfp);
+        const float path = rng_exponential(rng, rcp_fast(imfp));
         if (path < best) { best = path; ty = 2; }
     }
     {
         const float imfp = expf(tab1_get(m.in_imfp, le));
-        const float path = rng_exponential(rng, 1 / imfp);
+        const float path = rng_exponential(rng, rcp_fast(imfp));
         if (path < best) { best = path; ty = 1; }
     }
     *type = ty;
@@ -614,11 +619,14 @@ __device__ __forceinline__ bool scatter_begin_v(const dmat& m, dparticle& p, R& 
         }
         const float U = rng.unit();
         if ((w0 > 0) && (omega_min > w0) && (omega_min < omega_max)) {
-            const float f_min = 1 / w0 * logf((omega_min - w0) / omega_min);
-            const float f_max = 1 / w0 * logf((omega_max - w0) / omega_max);
-            omega = -w0 / expm1f(w0 * (f_min * (1 - U) + f_max * U));
+            // the quotients inside the logs stay correctly rounded: near 1,
+            // the log amplifies their error by omega_max / w0
+            const float iw0 = rcp_fast(w0);
+            const float f_min = iw0 * logf((omega_min - w0) / omega_min);
+            const float f_max = iw0 * logf((omega_max - w0) / omega_max);
+            omega = -w0 * rcp_fast(expm1f(w0 * (f_min * (1 - U) + f_max * U)));
         } else {
-            omega = omega0 * omega_max / (omega0 * (1 - U) + omega_max * U);
+            omega = omega0 * omega_max * rcp_fast(omega0 * (1 - U) + omega_max * U);
         }
     }
     if (binding < 0) {
@@ -667,11 +675,11 @@ __device__ __forceinline__ bool scatter_end(const dmat& m, dparticle& p, R& rng,
     const float binding = s.binding, omega = s.omega;
     const float K = p.e - m.fermi + 2 * binding;
     const float dK = binding + omega;
-    const float cos_theta = sqrt_fast(dK / K);
+    const float cos_theta = sqrt_fast(dK * rcp_fast(K));
     const float sin_theta = sqrt_fast(1 - clampr(cos_theta * cos_theta, 0, 1));
     f3 sdir = p.dir * cos_theta + nd * sin_theta;
     sdir = normalised_fast(sdir);
-    const float sb = sqrt_fast(binding / dK);
+    const float sb = sqrt_fast(binding * rcp_fast(dK));
     sdir = sdir + sb * rng_uniform_vector(rng);
     sdir = normalised_fast(sdir);
     sec->e = m.fermi + omega - binding;

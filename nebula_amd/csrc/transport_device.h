@@ -35,7 +35,7 @@ __device__ __forceinline__ bool init_sample(const dgeom& g, const dmat* __restri
         dist = sample_path(m, p.e, rng, &type);
         if (COUNT) c.lookups_1d += 2;
     }
-    p.dir = normalised(p.dir);
+    p.dir = normalised_fast(p.dir);      // rnorm3df in the reference's GPU build (gpu_driver.inl:458)
     return true;
 }
 
