@@ -60,9 +60,25 @@ struct __attribute__((aligned(16))) tri_rec {
 };
 static_assert(sizeof(tri_rec) == 48, "tri_rec is 48 bytes");
 
+// Side cone of a leaf, stored at the index of the leaf's first record (the
+// other entries are unused).  If every triangle of the leaf has the same
+// materials and the float normals cross(e1, e2) all lie within angle theta
+// of the unit axis a, then for a direction d with |d|^2 within 1e-4 of 1,
+// dot(a, d) > t implies that the reference's side test
+// fl(dot(cross(e1, e2), d)) < 0 is false for every triangle of the leaf,
+// and dot(a, d) < -t that it is true for every one: t = sin(theta + 1e-5)
+// (1 + 1e-4) + 1e-6 covers the rounding of both dot products (< 2e-7 |n| |d|
+// and < 4e-7) and the length of d.  t = 2 for leaves where this does not
+// hold (mixed materials, a near-degenerate triangle, a cone wider than
+// acos(0.2)).  Used to skip a leaf all of whose triangles the walk would
+// skip as facing the ignored material (octree.inl:128-138).
+struct __attribute__((aligned(16))) leaf_cone { float ax, ay, az, t; };
+static_assert(sizeof(leaf_cone) == 16, "leaf_cone is 16 bytes");
+
 struct dgeom {
     const uint64_t* nodes;
     const tri_rec* recs;
+    const leaf_cone* cone;     // per record index; valid at each leaf's first record
     const uint32_t* id_rec;    // input triangle id -> index of one of its records
     float cx, cy, cz;          // root centre
     float hx, hy, hz;          // root half size

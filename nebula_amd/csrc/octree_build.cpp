@@ -161,6 +161,40 @@ public:
     }
 };
 
+// The side cone of the leaf records r[0, n) (leaf_cone in nbl_device.h).  The
+// normals are the float cross products of the side test, so the cone holds
+// the vectors that test uses; the angles are measured in double against the
+// stored float axis.
+leaf_cone side_cone(const tri_rec* r, size_t n)
+{
+    const leaf_cone none{ 0, 0, 0, 2 };
+    std::vector<double> u(3 * n);
+    double sx = 0, sy = 0, sz = 0;
+    for (size_t k = 0; k < n; ++k) {
+        if (r[k].mats != r[0].mats) return none;
+        const f3 e1{ r[k].e1x, r[k].e1y, r[k].e1z }, e2{ r[k].e2x, r[k].e2y, r[k].e2z };
+        const f3 nf = cross(e1, e2);
+        const double x = nf.x, y = nf.y, z = nf.z;
+        const double len = std::sqrt(x * x + y * y + z * z);
+        const double l1 = std::sqrt((double)e1.x * e1.x + (double)e1.y * e1.y + (double)e1.z * e1.z);
+        const double l2 = std::sqrt((double)e2.x * e2.x + (double)e2.y * e2.y + (double)e2.z * e2.z);
+        if (!(len > 1e-3 * l1 * l2)) return none;     // near-degenerate (or not finite)
+        u[3 * k] = x / len; u[3 * k + 1] = y / len; u[3 * k + 2] = z / len;
+        sx += u[3 * k]; sy += u[3 * k + 1]; sz += u[3 * k + 2];
+    }
+    const double sl = std::sqrt(sx * sx + sy * sy + sz * sz);
+    if (!(sl > 0.5 * (double)n)) return none;
+    const float ax = (float)(sx / sl), ay = (float)(sy / sl), az = (float)(sz / sl);
+    const double al = std::sqrt((double)ax * ax + (double)ay * ay + (double)az * az);
+    double cmin = 1;
+    for (size_t k = 0; k < n; ++k)
+        cmin = std::min(cmin, (u[3 * k] * ax + u[3 * k + 1] * ay + u[3 * k + 2] * az) / al);
+    if (!(cmin >= 0.2)) return none;
+    const double theta = std::acos(std::min(cmin, 1.0)) + 1e-9;
+    const double t = std::sin(theta + 1e-5) * (1 + 1e-4) + 1e-6;
+    return leaf_cone{ ax, ay, az, std::nextafter((float)t, 2.0f) };
+}
+
 }  // namespace
 
 built_octree build_octree(const nbl_triangle* in, uint32_t n)
@@ -223,7 +257,8 @@ built_octree build_octree(const nbl_triangle* in, uint32_t n)
         if (b.kids >= 0) { word[id] = n_internal++; continue; }
         if (out.recs.size() + b.list.size() > 0xFFFFFFFFull)
             throw std::runtime_error("too many leaf triangle references");
-        word[id] = LEAF_BIT | ((uint64_t)b.list.size() << 32) | (uint64_t)out.recs.size();
+        const size_t start = out.recs.size();
+        word[id] = LEAF_BIT | ((uint64_t)b.list.size() << 32) | (uint64_t)start;
         ++out.n_leaves;
         for (uint32_t t : b.list) {
             const tri_in& s = tr.tris[t];
@@ -236,6 +271,8 @@ built_octree build_octree(const nbl_triangle* in, uint32_t n)
             out.recs.push_back(r);
             if (!seen[t]) { seen[t] = 1; ++out.n_unique; }
         }
+        out.cone.resize(out.recs.size(), leaf_cone{ 0, 0, 0, 2 });
+        if (!b.list.empty()) out.cone[start] = side_cone(&out.recs[start], b.list.size());
     }
     out.nodes.assign((size_t)n_internal * 8, 0);
     for (int64_t id : order) {

@@ -29,6 +29,21 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& tot
     return x - v;
 }
 
+// True when the side test (octree.inl:128-138) would skip every triangle of
+// the leaf whose records start at `first` for a ray along dir, so that the
+// leaf's candidates need not enter the shared tests: its side cone
+// (leaf_cone) puts dir clearly on one side of all the leaf's normals, and
+// the material on that side is the ignored one.
+__device__ __forceinline__ bool leaf_culled(const dgeom& g, uint32_t first, f3 dir, int ignore_mat)
+{
+    if (!(fabsf(dot(dir, dir) - 1) < 1e-4f)) return false;
+    const float4 cn = *reinterpret_cast<const float4*>(g.cone + first);
+    const uint32_t mats = g.recs[first].mats;
+    const float c = cn.x * dir.x + cn.y * dir.y + cn.z * dir.z;
+    const int mat_in = (int)(int16_t)(mats & 0xFFFFu), mat_out = (int)(int16_t)(mats >> 16);
+    return (c > cn.w && mat_out == ignore_mat) || (c < -cn.w && mat_in == ignore_mat);
+}
+
 // A lane's walk between cells (the state octree::propagate keeps across
 // iterations of its cell loop, octree.inl:32-190).
 struct walk_state {
@@ -63,6 +78,7 @@ __device__ __forceinline__ bool walk_cell(bool run, const dgeom& g, walk_state& 
         count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
         best = leaf_exit(c, s, w.pos, dir, axis);
         if (COUNT) cnt->node_steps += count + 1;
+        if (count > 0 && leaf_culled(g, first, dir, ignore_mat)) count = 0;
     }
     uint32_t total;
     const uint32_t off = wave_exclusive_sum(count, total);
