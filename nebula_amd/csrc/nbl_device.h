@@ -75,10 +75,25 @@ static_assert(sizeof(tri_rec) == 48, "tri_rec is 48 bytes");
 struct __attribute__((aligned(16))) leaf_cone { float ax, ay, az, t; };
 static_assert(sizeof(leaf_cone) == 16, "leaf_cone is 16 bytes");
 
+// Slab of a leaf along its cone axis a, stored like leaf_cone.  A walk whose
+// segment [pos, pos + T d] in the leaf, T = min(exit, distance left) +
+// EPSILON <= tmax, lies above hi or below lo (heights a.x in float) and
+// whose direction has |dot(a, d)| >= g cannot get a distance in (0, T] from
+// any of the leaf's triangles in the reference's float intersect_ray: every
+// triangle lies in [lo + m, hi - m] along a, and m bounds how far the float
+// hit point p + t d of intersect_ray can lie from the point of the triangle
+// given by its float barycentrics (its rounding error analysis, with the
+// triangles' shapes and sizes, the cell's size and position, and
+// |d . n| >= 0.02 |n| that g guarantees through the cone).  g = 2 where no
+// slab is kept.
+struct __attribute__((aligned(16))) leaf_slab { float lo, hi, g, tmax; };
+static_assert(sizeof(leaf_slab) == 16, "leaf_slab is 16 bytes");
+
 struct dgeom {
     const uint64_t* nodes;
     const tri_rec* recs;
     const leaf_cone* cone;     // per record index; valid at each leaf's first record
+    const leaf_slab* slab;     // the same for the leaf slabs
     const uint32_t* id_rec;    // input triangle id -> index of one of its records
     float cx, cy, cz;          // root centre
     float hx, hy, hz;          // root half size

@@ -161,38 +161,91 @@ public:
     }
 };
 
-// The side cone of the leaf records r[0, n) (leaf_cone in nbl_device.h).  The
-// normals are the float cross products of the side test, so the cone holds
-// the vectors that test uses; the angles are measured in double against the
-// stored float axis.
-leaf_cone side_cone(const tri_rec* r, size_t n)
+// Cull bounds of the leaf records r[0, n) in the cell bx: the side cone
+// (leaf_cone) and the slab (leaf_slab) of nbl_device.h.
+//
+// Cone: the normals are the float cross products of the side test, so the
+// cone holds the vectors that test uses; angles are measured in double
+// against the stored float axis a.
+//
+// Slab: with u = 2^-24, |d| = 1 and D = |det| = |n| g, g = |d . n|/|n|, the
+// reference's float intersect_ray (triangle.inl:7-28, no fused operations)
+// has |det_f - det| <= 7u |e1||e2|, and for tvec = pos - r0 (|tvec| <= R)
+// |un_f - un|, |vn_f - vn|, |tn_f - tn| <= 8u R |e2|, 8u R |e1|,
+// 8u R |e1||e2|; with 7u |e1||e2| <= D/20, |det_f| >= 0.75 D and
+//   |t_f - t| + |u_f - u||e1| + |v_f - v||e2|
+//     <= (K/g)(u/0.75)(24 R + 7.7 (T + |e1| + |e2|)) + 1.1u (T + |e1| + |e2|) = E,
+// K = |e1||e2|/|n|, for any hit the float tests accept (|u|, |v| < 1.1,
+// t <= 1.1 T).  The float hit point p + t_f d then lies within E of
+// r0 + u_f e1 + v_f e2, a point of the triangle grown by the EPSILON
+// tolerance (extent ext along a), so a segment beyond the slab grown by
+// 2E + ext, plus the rounding of the device's own heights, meets none.
+struct leaf_bounds_t { leaf_cone cone; leaf_slab slab; };
+
+leaf_bounds_t leaf_bounds(const tri_rec* r, size_t n, const box& bx)
 {
-    const leaf_cone none{ 0, 0, 0, 2 };
-    std::vector<double> u(3 * n);
+    const leaf_cone no_cone{ 0, 0, 0, 2 };
+    const leaf_slab no_slab{ 0, 0, 2, 0 };
+    std::vector<double> u(3 * n), kf(n);
     double sx = 0, sy = 0, sz = 0;
     for (size_t k = 0; k < n; ++k) {
-        if (r[k].mats != r[0].mats) return none;
+        if (r[k].mats != r[0].mats) return { no_cone, no_slab };
         const f3 e1{ r[k].e1x, r[k].e1y, r[k].e1z }, e2{ r[k].e2x, r[k].e2y, r[k].e2z };
         const f3 nf = cross(e1, e2);
         const double x = nf.x, y = nf.y, z = nf.z;
         const double len = std::sqrt(x * x + y * y + z * z);
         const double l1 = std::sqrt((double)e1.x * e1.x + (double)e1.y * e1.y + (double)e1.z * e1.z);
         const double l2 = std::sqrt((double)e2.x * e2.x + (double)e2.y * e2.y + (double)e2.z * e2.z);
-        if (!(len > 1e-3 * l1 * l2)) return none;     // near-degenerate (or not finite)
+        if (!(len > 1e-3 * l1 * l2)) return { no_cone, no_slab };   // near-degenerate (or not finite)
         u[3 * k] = x / len; u[3 * k + 1] = y / len; u[3 * k + 2] = z / len;
+        kf[k] = l1 * l2 / len;
         sx += u[3 * k]; sy += u[3 * k + 1]; sz += u[3 * k + 2];
     }
     const double sl = std::sqrt(sx * sx + sy * sy + sz * sz);
-    if (!(sl > 0.5 * (double)n)) return none;
+    if (!(sl > 0.5 * (double)n)) return { no_cone, no_slab };
     const float ax = (float)(sx / sl), ay = (float)(sy / sl), az = (float)(sz / sl);
     const double al = std::sqrt((double)ax * ax + (double)ay * ay + (double)az * az);
     double cmin = 1;
     for (size_t k = 0; k < n; ++k)
         cmin = std::min(cmin, (u[3 * k] * ax + u[3 * k + 1] * ay + u[3 * k + 2] * az) / al);
-    if (!(cmin >= 0.2)) return none;
+    if (!(cmin >= 0.2)) return { no_cone, no_slab };
     const double theta = std::acos(std::min(cmin, 1.0)) + 1e-9;
-    const double t = std::sin(theta + 1e-5) * (1 + 1e-4) + 1e-6;
-    return leaf_cone{ ax, ay, az, std::nextafter((float)t, 2.0f) };
+    const double tc = std::sin(theta + 1e-5) * (1 + 1e-4) + 1e-6;
+    const leaf_cone cone{ ax, ay, az, std::nextafter((float)tc, 2.0f) };
+
+    // slab
+    const double U = 0x1p-24, g_min = 0.02;
+    const double phi = std::acos(g_min) - theta;      // |angle(a, d)| <= phi  =>  g >= g_min
+    if (!(phi > 1e-3)) return { cone, no_slab };
+    const double half = std::sqrt((double)bx.h[0] * bx.h[0] + (double)bx.h[1] * bx.h[1] + (double)bx.h[2] * bx.h[2]);
+    const double cn = std::sqrt((double)bx.c[0] * bx.c[0] + (double)bx.c[1] * bx.c[1] + (double)bx.c[2] * bx.c[2]);
+    const double tmax = 2.03 * ((double)bx.h[0] + bx.h[1] + bx.h[2]) + 1e-3;
+    double lo = HUGE_VAL, hi = -HUGE_VAL, err = 0, ext = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const double r0[3] = { r[k].r0x, r[k].r0y, r[k].r0z };
+        const double e1[3] = { r[k].e1x, r[k].e1y, r[k].e1z }, e2[3] = { r[k].e2x, r[k].e2y, r[k].e2z };
+        const double l1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+        const double l2 = std::sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+        if (!(7 * U * kf[k] / (0.75 * g_min) <= 0.05)) return { cone, no_slab };
+        const double R = std::sqrt((r0[0] - bx.c[0]) * (r0[0] - bx.c[0]) + (r0[1] - bx.c[1]) * (r0[1] - bx.c[1]) +
+                                   (r0[2] - bx.c[2]) * (r0[2] - bx.c[2])) + half + 1e-3;
+        const double L = l1 + l2;
+        err = std::max(err, kf[k] / g_min * U / 0.75 * (24 * R + 7.7 * (tmax + L)) + 1.1 * U * (tmax + L));
+        ext = std::max(ext, 3 * (EPSILON + 3e-7) * L);
+        for (int v = 0; v < 3; ++v) {
+            const double p[3] = { r0[0] + (v == 1 ? e1[0] : v == 2 ? e2[0] : 0), r0[1] + (v == 1 ? e1[1] : v == 2 ? e2[1] : 0),
+                                  r0[2] + (v == 1 ? e1[2] : v == 2 ? e2[2] : 0) };
+            const double h = p[0] * ax + p[1] * ay + p[2] * az;
+            lo = std::min(lo, h); hi = std::max(hi, h);
+        }
+    }
+    if (!(std::isfinite(lo) && std::isfinite(hi))) return { cone, no_slab };
+    // the device's heights fl(a . pos) and fl(h0 + fl(T fl(a . d)))
+    const double eh = 8 * U * (cn + half + tmax + 1);
+    const double delta = 2 * err + ext + eh + 1e-6;
+    const double g = std::cos(phi) + 1e-4;            // covers fl(a . d) and |d| within 5e-5 of 1
+    return { cone, leaf_slab{ std::nextafter((float)(lo - delta), -HUGE_VALF), std::nextafter((float)(hi + delta), HUGE_VALF),
+                              std::nextafter((float)g, 2.0f), std::nextafter((float)tmax, 0.0f) } };
 }
 
 }  // namespace
@@ -243,9 +296,15 @@ built_octree build_octree(const nbl_triangle* in, uint32_t n)
     std::vector<int64_t> order;
     order.reserve(tr.nodes.size());
     order.push_back(0);
+    std::vector<box> boxes(tr.nodes.size());          // cell of every node
+    boxes[0] = root;
     for (size_t q = 0; q < order.size(); ++q) {
         const int64_t k = tr.nodes[order[q]].kids;
-        if (k >= 0) for (int o = 0; o < 8; ++o) order.push_back(k + o);
+        if (k >= 0)
+            for (int o = 0; o < 8; ++o) {
+                order.push_back(k + o);
+                boxes[k + o] = child_box(boxes[order[q]], o);
+            }
     }
     std::vector<uint64_t> word(tr.nodes.size(), 0);   // child word of every node
     built_octree out;
@@ -272,7 +331,12 @@ built_octree build_octree(const nbl_triangle* in, uint32_t n)
             if (!seen[t]) { seen[t] = 1; ++out.n_unique; }
         }
         out.cone.resize(out.recs.size(), leaf_cone{ 0, 0, 0, 2 });
-        if (!b.list.empty()) out.cone[start] = side_cone(&out.recs[start], b.list.size());
+        out.slab.resize(out.recs.size(), leaf_slab{ 0, 0, 2, 0 });
+        if (!b.list.empty()) {
+            const leaf_bounds_t lb = leaf_bounds(&out.recs[start], b.list.size(), boxes[id]);
+            out.cone[start] = lb.cone;
+            out.slab[start] = lb.slab;
+        }
     }
     out.nodes.assign((size_t)n_internal * 8, 0);
     for (int64_t id : order) {

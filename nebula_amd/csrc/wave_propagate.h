@@ -29,19 +29,39 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& tot
     return x - v;
 }
 
-// True when the side test (octree.inl:128-138) would skip every triangle of
-// the leaf whose records start at `first` for a ray along dir, so that the
-// leaf's candidates need not enter the shared tests: its side cone
-// (leaf_cone) puts dir clearly on one side of all the leaf's normals, and
-// the material on that side is the ignored one.
-__device__ __forceinline__ bool leaf_culled(const dgeom& g, uint32_t first, f3 dir, int ignore_mat)
+// True when no triangle of the leaf whose records start at `first` can take
+// part in the walk of this lane (so its candidates need not enter the shared
+// tests); the lane's direction has |d|^2 within 1e-4 of 1, and either
+//   * side cull: its side cone (leaf_cone) puts d clearly on one side of all
+//     the leaf's normals and the material on that side is the ignored one,
+//     so the side test (octree.inl:128-138) skips every triangle; or
+//   * slab cull: the segment [pos, pos + T d], T = min(exit, to_go) +
+//     EPSILON, lies outside the leaf's slab (leaf_slab), so intersect_ray
+//     returns no distance in (0, T] for any of them.  The walk keeps a
+//     distance only if it is in (0, exit + EPSILON], and when to_go <= exit
+//     one beyond to_go + EPSILON cannot change its result (the walk ends
+//     without a hit whenever such a distance would be kept), so dropping
+//     the leaf leaves the walk's result unchanged.
+// *tested = the candidates of a slab-culled leaf (all pass the side test:
+// g > t, so the side is decided and is not the ignored one).
+__device__ __forceinline__ bool leaf_dropped(const dgeom& g, uint32_t first, f3 pos, f3 dir, float exit,
+    float to_go, int ignore_mat, bool& slab_culled)
 {
+    slab_culled = false;
     if (!(fabsf(dot(dir, dir) - 1) < 1e-4f)) return false;
     const float4 cn = *reinterpret_cast<const float4*>(g.cone + first);
+    const float4 sl = *reinterpret_cast<const float4*>(g.slab + first);
     const uint32_t mats = g.recs[first].mats;
     const float c = cn.x * dir.x + cn.y * dir.y + cn.z * dir.z;
     const int mat_in = (int)(int16_t)(mats & 0xFFFFu), mat_out = (int)(int16_t)(mats >> 16);
-    return (c > cn.w && mat_out == ignore_mat) || (c < -cn.w && mat_in == ignore_mat);
+    if ((c > cn.w && mat_out == ignore_mat) || (c < -cn.w && mat_in == ignore_mat)) return true;
+    if (!(fabsf(c) >= sl.z)) return false;
+    const float T = fminf(exit, to_go) + EPSILON;
+    if (!(T <= sl.w)) return false;
+    const float h0 = cn.x * pos.x + cn.y * pos.y + cn.z * pos.z;
+    const float h1 = h0 + T * c;
+    slab_culled = (h0 > sl.y && h1 > sl.y) || (h0 < sl.x && h1 < sl.x);
+    return slab_culled;
 }
 
 // A lane's walk between cells (the state octree::propagate keeps across
@@ -78,7 +98,11 @@ __device__ __forceinline__ bool walk_cell(bool run, const dgeom& g, walk_state& 
         count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
         best = leaf_exit(c, s, w.pos, dir, axis);
         if (COUNT) cnt->node_steps += count + 1;
-        if (count > 0 && leaf_culled(g, first, dir, ignore_mat)) count = 0;
+        bool slab_culled;
+        if (count > 0 && leaf_dropped(g, first, w.pos, dir, best, w.to_go, ignore_mat, slab_culled)) {
+            if (COUNT && slab_culled) cnt->tri_tests += count;
+            count = 0;
+        }
     }
     uint32_t total;
     const uint32_t off = wave_exclusive_sum(count, total);
