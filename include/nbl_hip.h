@@ -414,7 +414,7 @@ typedef struct {
     uint32_t block_size;       /* threads per workgroup (64..1024, multiple of 64); 0 = default */
     uint32_t blocks_per_cu;    /* resident workgroups per CU; 0 = default          */
     uint32_t stack_depth;      /* per-wavefront secondary stack entries in LDS; 0 = default (128) */
-    uint32_t refill_threshold; /* refill a wavefront when >= this many lanes idle; 0 = default (1) */
+    uint32_t refill_threshold; /* refill a wavefront when >= this many lanes idle; 0 = default (4) */
     uint32_t overflow_depth;   /* further per-wavefront stack entries in HBM; 0 = default (4096) */
     uint32_t reserved[3];
 } nbl_persistent_opts;

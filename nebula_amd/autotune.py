@@ -30,7 +30,7 @@ DEFAULT_SPACE = {
     "block_size": (64, 128, 256),
     "blocks_per_cu": (2, 4, 8),
     "stack_depth": (64, 128),
-    "refill_threshold": (1, 8, 32),
+    "refill_threshold": (1, 4, 8, 32),
 }
 
 

@@ -429,7 +429,7 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
     // the learned variant holds 3 wavefronts per SIMD (12 per CU), the table variant 4
     uint32_t per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : (d->sampler ? 3u : 4u);
     uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 128;
-    uint32_t thresh = opts && opts->refill_threshold ? opts->refill_threshold : 1;
+    uint32_t thresh = opts && opts->refill_threshold ? opts->refill_threshold : 4;
     uint32_t ovf = opts && opts->overflow_depth ? opts->overflow_depth : 4096;
     require(ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
     require(block % 64 == 0 && block >= 64 && block <= 1024, "nbl_driver_run_persistent: block_size must be 64..1024, multiple of 64");
