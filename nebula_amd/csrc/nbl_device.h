@@ -60,8 +60,8 @@ struct __attribute__((aligned(16))) tri_rec {
 };
 static_assert(sizeof(tri_rec) == 48, "tri_rec is 48 bytes");
 
-// Side cone of a leaf, stored at the index of the leaf's first record (the
-// other entries are unused).  If every triangle of the leaf has the same
+// Side cone of a leaf, stored (in leaf_cull) at the index of the leaf's first
+// record (the other entries are unused).  If every triangle of the leaf has the same
 // materials and the float normals cross(e1, e2) all lie within angle theta
 // of the unit axis a, then for a direction d with |d|^2 within 1e-4 of 1,
 // dot(a, d) > t implies that the reference's side test
@@ -75,7 +75,7 @@ static_assert(sizeof(tri_rec) == 48, "tri_rec is 48 bytes");
 struct __attribute__((aligned(16))) leaf_cone { float ax, ay, az, t; };
 static_assert(sizeof(leaf_cone) == 16, "leaf_cone is 16 bytes");
 
-// Slab of a leaf along its cone axis a, stored like leaf_cone.  A walk whose
+// Slab of a leaf along its cone axis a, stored beside its leaf_cone.  A walk whose
 // segment [pos, pos + T d] in the leaf, T = min(exit, distance left) +
 // EPSILON <= tmax, lies above hi or below lo (heights a.x in float) and
 // whose direction has |dot(a, d)| >= g cannot get a distance in (0, T] from
@@ -89,11 +89,15 @@ static_assert(sizeof(leaf_cone) == 16, "leaf_cone is 16 bytes");
 struct __attribute__((aligned(16))) leaf_slab { float lo, hi, g, tmax; };
 static_assert(sizeof(leaf_slab) == 16, "leaf_slab is 16 bytes");
 
+// Both cull records of a leaf side by side: one 32-B aligned entry, one
+// cache line per leaf visit.
+struct __attribute__((aligned(32))) leaf_cull { leaf_cone cone; leaf_slab slab; };
+static_assert(sizeof(leaf_cull) == 32, "leaf_cull is 32 bytes");
+
 struct dgeom {
     const uint64_t* nodes;
     const tri_rec* recs;
-    const leaf_cone* cone;     // per record index; valid at each leaf's first record
-    const leaf_slab* slab;     // the same for the leaf slabs
+    const leaf_cull* cull;     // per record index; valid at each leaf's first record
     const uint32_t* id_rec;    // input triangle id -> index of one of its records
     float cx, cy, cz;          // root centre
     float hx, hy, hz;          // root half size

@@ -180,9 +180,7 @@ public:
 // r0 + u_f e1 + v_f e2, a point of the triangle grown by the EPSILON
 // tolerance (extent ext along a), so a segment beyond the slab grown by
 // 2E + ext, plus the rounding of the device's own heights, meets none.
-struct leaf_bounds_t { leaf_cone cone; leaf_slab slab; };
-
-leaf_bounds_t leaf_bounds(const tri_rec* r, size_t n, const box& bx)
+leaf_cull leaf_bounds(const tri_rec* r, size_t n, const box& bx)
 {
     const leaf_cone no_cone{ 0, 0, 0, 2 };
     const leaf_slab no_slab{ 0, 0, 2, 0 };
@@ -330,13 +328,8 @@ built_octree build_octree(const nbl_triangle* in, uint32_t n)
             out.recs.push_back(r);
             if (!seen[t]) { seen[t] = 1; ++out.n_unique; }
         }
-        out.cone.resize(out.recs.size(), leaf_cone{ 0, 0, 0, 2 });
-        out.slab.resize(out.recs.size(), leaf_slab{ 0, 0, 2, 0 });
-        if (!b.list.empty()) {
-            const leaf_bounds_t lb = leaf_bounds(&out.recs[start], b.list.size(), boxes[id]);
-            out.cone[start] = lb.cone;
-            out.slab[start] = lb.slab;
-        }
+        out.cull.resize(out.recs.size(), leaf_cull{ { 0, 0, 0, 2 }, { 0, 0, 2, 0 } });
+        if (!b.list.empty()) out.cull[start] = leaf_bounds(&out.recs[start], b.list.size(), boxes[id]);
     }
     out.nodes.assign((size_t)n_internal * 8, 0);
     for (int64_t id : order) {

@@ -12,8 +12,7 @@ namespace nbl {
 struct built_octree {
     std::vector<uint64_t> nodes;     // 8 child words per internal node, BFS order
     std::vector<tri_rec> recs;       // leaf records, BFS leaf order
-    std::vector<leaf_cone> cone;     // per record; each leaf's side cone at its first record
-    std::vector<leaf_slab> slab;     // per record; each leaf's slab at its first record
+    std::vector<leaf_cull> cull;     // per record; each leaf's cone and slab at its first record
     float center[3] = { 0, 0, 0 }, halfsize[3] = { 0, 0, 0 };
     float max_extent = 0;
     uint32_t n_unique = 0, n_leaves = 0, max_depth = 0;

@@ -49,8 +49,8 @@ __device__ __forceinline__ bool leaf_dropped(const dgeom& g, uint32_t first, f3 
 {
     slab_culled = false;
     if (!(fabsf(dot(dir, dir) - 1) < 1e-4f)) return false;
-    const float4 cn = *reinterpret_cast<const float4*>(g.cone + first);
-    const float4 sl = *reinterpret_cast<const float4*>(g.slab + first);
+    const float4* lc = reinterpret_cast<const float4*>(g.cull + first);
+    const float4 cn = lc[0], sl = lc[1];
     const uint32_t mats = g.recs[first].mats;
     const float c = cn.x * dir.x + cn.y * dir.y + cn.z * dir.z;
     const int mat_in = (int)(int16_t)(mats & 0xFFFFu), mat_out = (int)(int16_t)(mats >> 16);

@@ -105,7 +105,7 @@ int main(int argc, char** argv)
         const uint32_t first = (uint32_t)w, cnt = (uint32_t)((w >> 32) & 0x7FFFFFFFu);
         if (cnt == 0) continue;
         cand += cnt;
-        leaf_cone c = o.cone[first];
+        leaf_cone c = o.cull[first].cone;
         if (!(c.t < 2)) continue;
         cand_cone += cnt;
         c.t *= cone_scale;
@@ -134,7 +134,7 @@ int main(int argc, char** argv)
             }
         }
 
-        leaf_slab sl = o.slab[first];
+        leaf_slab sl = o.cull[first].slab;
         if (!(sl.g < 2)) continue;
         cand_slab += cnt;
         {
