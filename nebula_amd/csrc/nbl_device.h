@@ -44,8 +44,9 @@ NBL_HD f3 normalised(f3 a) { const float r = 1 / sqrtf(a.x * a.x + a.y * a.y + a
 // Geometry layout in HBM
 //   nodes : 8 x uint64 child words per internal octree node (one 64-B line);
 //           node 0 is the root.  A child word is either an internal node
-//           index (bit 63 clear) or a leaf (bit 63 set) with the leaf's first
-//           record in bits [0,32) and its record count in bits [32,63).
+//           index (bit 63 clear) or a leaf (bit 63 set) with its record
+//           count in bits [32,63) and, if that is not 0, the index of its
+//           leaf_info (first record, cull bounds) in bits [0,32).
 //   recs  : 48-B triangle records, stored once per leaf reference, in the
 //           leaf's triangle order, so a leaf is one contiguous run.
 // ---------------------------------------------------------------------------
@@ -60,8 +61,11 @@ struct __attribute__((aligned(16))) tri_rec {
 };
 static_assert(sizeof(tri_rec) == 48, "tri_rec is 48 bytes");
 
-// Side cone of a leaf, stored (in leaf_cull) at the index of the leaf's first
-// record (the other entries are unused).  If every triangle of the leaf has the same
+// Cull record of a non-empty leaf (leaf_info), in a dense array indexed by
+// the leaf's child word; it also holds the leaf's first record and its
+// materials (every triangle's, when they agree).
+//
+// Side cone (ax, ay, az, t): if every triangle of the leaf has the same
 // materials and the float normals cross(e1, e2) all lie within angle theta
 // of the unit axis a, then for a direction d with |d|^2 within 1e-4 of 1,
 // dot(a, d) > t implies that the reference's side test
@@ -72,32 +76,31 @@ static_assert(sizeof(tri_rec) == 48, "tri_rec is 48 bytes");
 // hold (mixed materials, a near-degenerate triangle, a cone wider than
 // acos(0.2)).  Used to skip a leaf all of whose triangles the walk would
 // skip as facing the ignored material (octree.inl:128-138).
-struct __attribute__((aligned(16))) leaf_cone { float ax, ay, az, t; };
-static_assert(sizeof(leaf_cone) == 16, "leaf_cone is 16 bytes");
-
-// Slab of a leaf along its cone axis a, stored beside its leaf_cone.  A walk whose
-// segment [pos, pos + T d] in the leaf, T = min(exit, distance left) +
-// EPSILON <= tmax, lies above hi or below lo (heights a.x in float) and
-// whose direction has |dot(a, d)| >= g cannot get a distance in (0, T] from
-// any of the leaf's triangles in the reference's float intersect_ray: every
-// triangle lies in [lo + m, hi - m] along a, and m bounds how far the float
-// hit point p + t d of intersect_ray can lie from the point of the triangle
-// given by its float barycentrics (its rounding error analysis, with the
-// triangles' shapes and sizes, the cell's size and position, and
-// |d . n| >= 0.02 |n| that g guarantees through the cone).  g = 2 where no
-// slab is kept.
-struct __attribute__((aligned(16))) leaf_slab { float lo, hi, g, tmax; };
-static_assert(sizeof(leaf_slab) == 16, "leaf_slab is 16 bytes");
-
-// Both cull records of a leaf side by side: one 32-B aligned entry, one
-// cache line per leaf visit.
-struct __attribute__((aligned(32))) leaf_cull { leaf_cone cone; leaf_slab slab; };
-static_assert(sizeof(leaf_cull) == 32, "leaf_cull is 32 bytes");
+//
+// Slab (lo, hi) along a: a walk whose segment [pos, pos + T d] in the leaf,
+// T = min(exit, distance left) + EPSILON <= 2.03 (sx + sy + sz) + 1e-3 (s =
+// the cell's half sizes), lies above hi or below lo (heights a.x in float)
+// and whose direction has |dot(a, d)| >= t + 0.0202 cannot get a distance
+// in (0, T] from any of the leaf's triangles in the reference's float
+// intersect_ray: every triangle lies in [lo + m, hi - m] along a, and m
+// bounds how far the float hit point p + t d of intersect_ray can lie from
+// the point of the triangle given by its float barycentrics (its rounding
+// error analysis, with the triangles' shapes and sizes, the cell's size and
+// position, and |d . n| >= 0.02 |n|, which |dot(a, d)| >= t + 0.0202
+// guarantees through the cone since sin(theta + asin 0.02) <= t + 0.0201).
+// lo = -inf, hi = +inf where no slab is kept.
+struct __attribute__((aligned(32))) leaf_info {
+    float ax, ay, az, t;       // side cone
+    float lo, hi;              // slab
+    uint32_t first;            // the leaf's first record
+    uint32_t mats;             // its triangles' materials (as tri_rec::mats) when t < 2
+};
+static_assert(sizeof(leaf_info) == 32, "leaf_info is 32 bytes");
 
 struct dgeom {
     const uint64_t* nodes;
     const tri_rec* recs;
-    const leaf_cull* cull;     // per record index; valid at each leaf's first record
+    const leaf_info* leaves;   // per non-empty leaf, indexed by its child word
     const uint32_t* id_rec;    // input triangle id -> index of one of its records
     float cx, cy, cz;          // root centre
     float hx, hy, hz;          // root half size
@@ -373,8 +376,8 @@ __device__ __forceinline__ float propagate(const dgeom& g, f3 start, f3 dir, flo
     do {
         f3 c, s;
         const uint64_t child = descend<COUNT>(g, pos, location, c, s, cnt, pc);
-        const uint32_t first = (uint32_t)child;
         const uint32_t count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
+        const uint32_t first = count ? g.leaves[(uint32_t)child].first : 0u;
         int axis;          // -1, -2, -4 for x, y, z; 0 once a triangle is kept
         float best = leaf_exit(c, s, pos, dir, axis);
         uint32_t best_id = NO_TRI;

@@ -161,8 +161,8 @@ public:
     }
 };
 
-// Cull bounds of the leaf records r[0, n) in the cell bx: the side cone
-// (leaf_cone) and the slab (leaf_slab) of nbl_device.h.
+// Cull bounds of the leaf records r[0, n) in the cell bx: the side cone and
+// the slab of its leaf_info (nbl_device.h); first and mats are the caller's.
 //
 // Cone: the normals are the float cross products of the side test, so the
 // cone holds the vectors that test uses; angles are measured in double
@@ -180,51 +180,52 @@ public:
 // r0 + u_f e1 + v_f e2, a point of the triangle grown by the EPSILON
 // tolerance (extent ext along a), so a segment beyond the slab grown by
 // 2E + ext, plus the rounding of the device's own heights, meets none.
-leaf_cull leaf_bounds(const tri_rec* r, size_t n, const box& bx)
+leaf_info leaf_bounds(const tri_rec* r, size_t n, const box& bx)
 {
-    const leaf_cone no_cone{ 0, 0, 0, 2 };
-    const leaf_slab no_slab{ 0, 0, 2, 0 };
+    const float inf = HUGE_VALF;
+    const leaf_info none{ 0, 0, 0, 2, -inf, inf, 0, 0 };
     std::vector<double> u(3 * n), kf(n);
     double sx = 0, sy = 0, sz = 0;
     for (size_t k = 0; k < n; ++k) {
-        if (r[k].mats != r[0].mats) return { no_cone, no_slab };
+        if (r[k].mats != r[0].mats) return none;
         const f3 e1{ r[k].e1x, r[k].e1y, r[k].e1z }, e2{ r[k].e2x, r[k].e2y, r[k].e2z };
         const f3 nf = cross(e1, e2);
         const double x = nf.x, y = nf.y, z = nf.z;
         const double len = std::sqrt(x * x + y * y + z * z);
         const double l1 = std::sqrt((double)e1.x * e1.x + (double)e1.y * e1.y + (double)e1.z * e1.z);
         const double l2 = std::sqrt((double)e2.x * e2.x + (double)e2.y * e2.y + (double)e2.z * e2.z);
-        if (!(len > 1e-3 * l1 * l2)) return { no_cone, no_slab };   // near-degenerate (or not finite)
+        if (!(len > 1e-3 * l1 * l2)) return none;   // near-degenerate (or not finite)
         u[3 * k] = x / len; u[3 * k + 1] = y / len; u[3 * k + 2] = z / len;
         kf[k] = l1 * l2 / len;
         sx += u[3 * k]; sy += u[3 * k + 1]; sz += u[3 * k + 2];
     }
     const double sl = std::sqrt(sx * sx + sy * sy + sz * sz);
-    if (!(sl > 0.5 * (double)n)) return { no_cone, no_slab };
+    if (!(sl > 0.5 * (double)n)) return none;
     const float ax = (float)(sx / sl), ay = (float)(sy / sl), az = (float)(sz / sl);
     const double al = std::sqrt((double)ax * ax + (double)ay * ay + (double)az * az);
     double cmin = 1;
     for (size_t k = 0; k < n; ++k)
         cmin = std::min(cmin, (u[3 * k] * ax + u[3 * k + 1] * ay + u[3 * k + 2] * az) / al);
-    if (!(cmin >= 0.2)) return { no_cone, no_slab };
+    if (!(cmin >= 0.2)) return none;
     const double theta = std::acos(std::min(cmin, 1.0)) + 1e-9;
     const double tc = std::sin(theta + 1e-5) * (1 + 1e-4) + 1e-6;
-    const leaf_cone cone{ ax, ay, az, std::nextafter((float)tc, 2.0f) };
+    const leaf_info cone{ ax, ay, az, std::nextafter((float)tc, 2.0f), -inf, inf, 0, 0 };
 
     // slab
+    // the device's |fl(a . d)| >= t + 0.0202 gives a . d >= sin(theta + asin g_min)
+    // for |d| within 5e-5 of 1, so every |d . n| >= g_min |n|
     const double U = 0x1p-24, g_min = 0.02;
-    const double phi = std::acos(g_min) - theta;      // |angle(a, d)| <= phi  =>  g >= g_min
-    if (!(phi > 1e-3)) return { cone, no_slab };
     const double half = std::sqrt((double)bx.h[0] * bx.h[0] + (double)bx.h[1] * bx.h[1] + (double)bx.h[2] * bx.h[2]);
     const double cn = std::sqrt((double)bx.c[0] * bx.c[0] + (double)bx.c[1] * bx.c[1] + (double)bx.c[2] * bx.c[2]);
-    const double tmax = 2.03 * ((double)bx.h[0] + bx.h[1] + bx.h[2]) + 1e-3;
+    // bounds the device's segment limit fl(2.03 fl(sx + sy + sz) + 1e-3)
+    const double tmax = 2.04 * ((double)bx.h[0] + bx.h[1] + bx.h[2]) + 1.01e-3;
     double lo = HUGE_VAL, hi = -HUGE_VAL, err = 0, ext = 0;
     for (size_t k = 0; k < n; ++k) {
         const double r0[3] = { r[k].r0x, r[k].r0y, r[k].r0z };
         const double e1[3] = { r[k].e1x, r[k].e1y, r[k].e1z }, e2[3] = { r[k].e2x, r[k].e2y, r[k].e2z };
         const double l1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
         const double l2 = std::sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
-        if (!(7 * U * kf[k] / (0.75 * g_min) <= 0.05)) return { cone, no_slab };
+        if (!(7 * U * kf[k] / (0.75 * g_min) <= 0.05)) return cone;
         const double R = std::sqrt((r0[0] - bx.c[0]) * (r0[0] - bx.c[0]) + (r0[1] - bx.c[1]) * (r0[1] - bx.c[1]) +
                                    (r0[2] - bx.c[2]) * (r0[2] - bx.c[2])) + half + 1e-3;
         const double L = l1 + l2;
@@ -237,13 +238,14 @@ leaf_cull leaf_bounds(const tri_rec* r, size_t n, const box& bx)
             lo = std::min(lo, h); hi = std::max(hi, h);
         }
     }
-    if (!(std::isfinite(lo) && std::isfinite(hi))) return { cone, no_slab };
+    if (!(std::isfinite(lo) && std::isfinite(hi))) return cone;
     // the device's heights fl(a . pos) and fl(h0 + fl(T fl(a . d)))
     const double eh = 8 * U * (cn + half + tmax + 1);
     const double delta = 2 * err + ext + eh + 1e-6;
-    const double g = std::cos(phi) + 1e-4;            // covers fl(a . d) and |d| within 5e-5 of 1
-    return { cone, leaf_slab{ std::nextafter((float)(lo - delta), -HUGE_VALF), std::nextafter((float)(hi + delta), HUGE_VALF),
-                              std::nextafter((float)g, 2.0f), std::nextafter((float)tmax, 0.0f) } };
+    leaf_info res = cone;
+    res.lo = std::nextafter((float)(lo - delta), -inf);
+    res.hi = std::nextafter((float)(hi + delta), inf);
+    return res;
 }
 
 }  // namespace
@@ -315,7 +317,7 @@ built_octree build_octree(const nbl_triangle* in, uint32_t n)
         if (out.recs.size() + b.list.size() > 0xFFFFFFFFull)
             throw std::runtime_error("too many leaf triangle references");
         const size_t start = out.recs.size();
-        word[id] = LEAF_BIT | ((uint64_t)b.list.size() << 32) | (uint64_t)start;
+        word[id] = LEAF_BIT | ((uint64_t)b.list.size() << 32) | (b.list.empty() ? 0ull : (uint64_t)out.leaves.size());
         ++out.n_leaves;
         for (uint32_t t : b.list) {
             const tri_in& s = tr.tris[t];
@@ -328,8 +330,12 @@ built_octree build_octree(const nbl_triangle* in, uint32_t n)
             out.recs.push_back(r);
             if (!seen[t]) { seen[t] = 1; ++out.n_unique; }
         }
-        out.cull.resize(out.recs.size(), leaf_cull{ { 0, 0, 0, 2 }, { 0, 0, 2, 0 } });
-        if (!b.list.empty()) out.cull[start] = leaf_bounds(&out.recs[start], b.list.size(), boxes[id]);
+        if (!b.list.empty()) {
+            leaf_info li = leaf_bounds(&out.recs[start], b.list.size(), boxes[id]);
+            li.first = (uint32_t)start;
+            li.mats = out.recs[start].mats;
+            out.leaves.push_back(li);
+        }
     }
     out.nodes.assign((size_t)n_internal * 8, 0);
     for (int64_t id : order) {
@@ -361,7 +367,7 @@ void export_linearized(const built_octree& o, std::vector<int32_t>& data, std::v
     std::vector<int32_t> tmap;
     for (const item& it : bfs) {
         if (!(it.w & LEAF_BIT)) continue;
-        const uint32_t first = (uint32_t)it.w, cnt = (uint32_t)((it.w >> 32) & 0x7FFFFFFFu);
+        const uint32_t first = leaf_first(o, it.w), cnt = (uint32_t)((it.w >> 32) & 0x7FFFFFFFu);
         for (uint32_t k = first; k < first + cnt; ++k) {
             const uint32_t id = o.recs[k].id;
             if (id >= tmap.size()) tmap.resize(id + 1, -1);
@@ -380,7 +386,7 @@ void export_linearized(const built_octree& o, std::vector<int32_t>& data, std::v
         offset[i] = (int64_t)data.size();
         const uint64_t w = bfs[i].w;
         if (w & LEAF_BIT) {
-            const uint32_t first = (uint32_t)w, cnt = (uint32_t)((w >> 32) & 0x7FFFFFFFu);
+            const uint32_t first = leaf_first(o, w), cnt = (uint32_t)((w >> 32) & 0x7FFFFFFFu);
             for (uint32_t k = first; k < first + cnt; ++k) data.push_back(tmap[o.recs[k].id]);
             data.push_back(-1);
         } else {

@@ -29,39 +29,46 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& tot
     return x - v;
 }
 
-// True when no triangle of the leaf whose records start at `first` can take
-// part in the walk of this lane (so its candidates need not enter the shared
-// tests); the lane's direction has |d|^2 within 1e-4 of 1, and either
-//   * side cull: its side cone (leaf_cone) puts d clearly on one side of all
-//     the leaf's normals and the material on that side is the ignored one,
-//     so the side test (octree.inl:128-138) skips every triangle; or
+// True when no triangle of the leaf (its leaf_info li, cell half sizes s)
+// can take part in the walk of this lane, so that its candidates need not
+// enter the shared tests; the lane's direction has |d|^2 within 1e-4 of 1,
+// and either
+//   * side cull: the side cone puts d clearly on one side of all the leaf's
+//     normals and the material on that side is the ignored one, so the side
+//     test (octree.inl:128-138) skips every triangle; or
 //   * slab cull: the segment [pos, pos + T d], T = min(exit, to_go) +
-//     EPSILON, lies outside the leaf's slab (leaf_slab), so intersect_ray
-//     returns no distance in (0, T] for any of them.  The walk keeps a
-//     distance only if it is in (0, exit + EPSILON], and when to_go <= exit
-//     one beyond to_go + EPSILON cannot change its result (the walk ends
-//     without a hit whenever such a distance would be kept), so dropping
-//     the leaf leaves the walk's result unchanged.
-// *tested = the candidates of a slab-culled leaf (all pass the side test:
-// g > t, so the side is decided and is not the ignored one).
-__device__ __forceinline__ bool leaf_dropped(const dgeom& g, uint32_t first, f3 pos, f3 dir, float exit,
-    float to_go, int ignore_mat, bool& slab_culled)
+//     EPSILON, lies outside the leaf's slab, so intersect_ray returns no
+//     distance in (0, T] for any of them.  The walk keeps a distance only if
+//     it is in (0, exit + EPSILON], and when to_go <= exit one beyond
+//     to_go + EPSILON cannot change its result (the walk ends without a hit
+//     whenever such a distance would be kept), so dropping the leaf leaves
+//     the walk's result unchanged.
+// slab_culled: the slab cull applied; then all the leaf's candidates pass
+// the side test (|dot(a, d)| > t: the side is decided and is not the
+// ignored one), and the reference tests each of them.
+__device__ __forceinline__ bool leaf_dropped(const leaf_info& li, f3 s, f3 pos, f3 dir, float exit, float to_go,
+    int ignore_mat, bool& slab_culled)
 {
     slab_culled = false;
     if (!(fabsf(dot(dir, dir) - 1) < 1e-4f)) return false;
-    const float4* lc = reinterpret_cast<const float4*>(g.cull + first);
-    const float4 cn = lc[0], sl = lc[1];
-    const uint32_t mats = g.recs[first].mats;
-    const float c = cn.x * dir.x + cn.y * dir.y + cn.z * dir.z;
-    const int mat_in = (int)(int16_t)(mats & 0xFFFFu), mat_out = (int)(int16_t)(mats >> 16);
-    if ((c > cn.w && mat_out == ignore_mat) || (c < -cn.w && mat_in == ignore_mat)) return true;
-    if (!(fabsf(c) >= sl.z)) return false;
+    const float c = li.ax * dir.x + li.ay * dir.y + li.az * dir.z;
+    const int mat_in = (int)(int16_t)(li.mats & 0xFFFFu), mat_out = (int)(int16_t)(li.mats >> 16);
+    if ((c > li.t && mat_out == ignore_mat) || (c < -li.t && mat_in == ignore_mat)) return true;
+    if (!(fabsf(c) >= li.t + 0.0202f)) return false;
     const float T = fminf(exit, to_go) + EPSILON;
-    if (!(T <= sl.w)) return false;
-    const float h0 = cn.x * pos.x + cn.y * pos.y + cn.z * pos.z;
+    if (!(T <= 2.03f * (s.x + s.y + s.z) + 1e-3f)) return false;
+    const float h0 = li.ax * pos.x + li.ay * pos.y + li.az * pos.z;
     const float h1 = h0 + T * c;
-    slab_culled = (h0 > sl.y && h1 > sl.y) || (h0 < sl.x && h1 < sl.x);
+    slab_culled = (h0 > li.hi && h1 > li.hi) || (h0 < li.lo && h1 < li.lo);
     return slab_culled;
+}
+
+// leaf_info k as two 16-B loads
+__device__ __forceinline__ leaf_info load_leaf(const dgeom& g, uint32_t k)
+{
+    const float4* p = reinterpret_cast<const float4*>(g.leaves + k);
+    const float4 a = p[0], b = p[1];
+    return leaf_info{ a.x, a.y, a.z, a.w, b.x, b.y, __float_as_uint(b.z), __float_as_uint(b.w) };
 }
 
 // A lane's walk between cells (the state octree::propagate keeps across
@@ -94,14 +101,17 @@ __device__ __forceinline__ bool walk_cell(bool run, const dgeom& g, walk_state& 
     if (run) {
         f3 c, s;
         const uint64_t child = descend<COUNT>(g, w.pos, w.location, c, s, cnt, pc);
-        first = (uint32_t)child;
         count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
         best = leaf_exit(c, s, w.pos, dir, axis);
         if (COUNT) cnt->node_steps += count + 1;
-        bool slab_culled;
-        if (count > 0 && leaf_dropped(g, first, w.pos, dir, best, w.to_go, ignore_mat, slab_culled)) {
-            if (COUNT && slab_culled) cnt->tri_tests += count;
-            count = 0;
+        if (count > 0) {
+            const leaf_info li = load_leaf(g, (uint32_t)child);
+            first = li.first;
+            bool slab_culled;
+            if (leaf_dropped(li, s, w.pos, dir, best, w.to_go, ignore_mat, slab_culled)) {
+                if (COUNT && slab_culled) cnt->tri_tests += count;
+                count = 0;
+            }
         }
     }
     uint32_t total;

@@ -1,7 +1,8 @@
 // leaf_cull_check.cpp -- soundness of the octree's leaf culls on one scene
 // (built by tests/test_leaf_culls.py with hipcc, run on the CPU).
 //
-// The device walk drops a leaf's candidates (wave_propagate.h: leaf_dropped)
+// The device walk drops a leaf's candidates (wave_propagate.h: leaf_dropped,
+// with the leaf's leaf_info of nebula_amd/csrc/nbl_device.h)
 //   * side cull: when dot(axis, d) > t (or < -t) and the material on that
 //     side is the ignored one -- exact only if then every triangle's float
 //     side test fl(dot(cross(e1, e2), d)) < 0 of the reference
@@ -102,12 +103,14 @@ int main(int argc, char** argv)
     };
     size_t cand = 0, cand_cone = 0, cand_slab = 0, checks = 0, bad = 0, seg = 0, seg_bad = 0;
     for (const auto& [w, cl] : leaves) {
-        const uint32_t first = (uint32_t)w, cnt = (uint32_t)((w >> 32) & 0x7FFFFFFFu);
+        const uint32_t cnt = (uint32_t)((w >> 32) & 0x7FFFFFFFu);
         if (cnt == 0) continue;
         cand += cnt;
-        leaf_cone c = o.cull[first].cone;
-        if (!(c.t < 2)) continue;
+        const leaf_info& li = o.leaves[(uint32_t)w];
+        const uint32_t first = li.first;
+        if (!(li.t < 2)) continue;
         cand_cone += cnt;
+        leaf_info c = li;
         c.t *= cone_scale;
         const double al = std::sqrt((double)c.ax * c.ax + (double)c.ay * c.ay + (double)c.az * c.az);
         const double ax = c.ax / al, ay = c.ay / al, az = c.az / al;
@@ -134,12 +137,16 @@ int main(int argc, char** argv)
             }
         }
 
-        leaf_slab sl = o.cull[first].slab;
-        if (!(sl.g < 2)) continue;
+        if (!(li.lo > -HUGE_VALF)) continue;
         cand_slab += cnt;
+        struct { float lo, hi, g, tmax; } sl;
         {
-            const float mid = 0.5f * (sl.lo + sl.hi), hw = 0.5f * (sl.hi - sl.lo) * slab_scale;
+            const float mid = 0.5f * (li.lo + li.hi), hw = 0.5f * (li.hi - li.lo) * slab_scale;
             sl.lo = mid - hw; sl.hi = mid + hw;
+            // the device's limits on direction and segment length
+            sl.g = li.t + 0.0202f;
+            const float sx = (float)cl.h[0], sy = (float)cl.h[1], sz = (float)cl.h[2];
+            sl.tmax = 2.03f * (sx + sy + sz) + 1e-3f;
         }
         for (int s = 0; s < draws; ++s) {
             const bool above = s & 1;
@@ -167,9 +174,9 @@ int main(int argc, char** argv)
                         (float)(len * (sg * cc * az + sn * qz)) };
             // the device's decision (wave_propagate.h: leaf_dropped)
             if (!(std::fabs(dot(d, d) - 1) < 1e-4f)) continue;
-            const float cf = c.ax * d.x + c.ay * d.y + c.az * d.z;
+            const float cf = li.ax * d.x + li.ay * d.y + li.az * d.z;
             if (!(std::fabs(cf) >= sl.g)) continue;
-            const float h0 = c.ax * pos.x + c.ay * pos.y + c.az * pos.z;
+            const float h0 = li.ax * pos.x + li.ay * pos.y + li.az * pos.z;
             // length: up to the slab when heading for it, else anything up to tmax
             float T = sl.tmax * (float)U(rng);
             if ((above && cf < 0) || (!above && cf > 0)) {
