@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--stack-depth", type=int, default=0)
     ap.add_argument("--refill-threshold", type=int, default=0)
+    ap.add_argument("--inelastic-batch", type=int, default=0)
+    ap.add_argument("--inelastic-wait", type=int, default=0)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
                     help="size of the plain copy kernel run first (counter scale, bandwidth yardstick); 0 = skip")
     args = ap.parse_args()
@@ -236,7 +238,8 @@ def main():
               file=sys.stderr, flush=True)
 
     opts = dict(block_size=args.block_size, blocks_per_cu=args.blocks_per_cu,
-                stack_depth=args.stack_depth, refill_threshold=args.refill_threshold)
+                stack_depth=args.stack_depth, refill_threshold=args.refill_threshold,
+                inelastic_batch=args.inelastic_batch, inelastic_wait=args.inelastic_wait)
     drv = Driver(g, m, capacity=1024, seed=2024)
     beam = lambda s: dict(base_beam, seed=s)
     for w in range(args.warmup):

@@ -14,6 +14,11 @@
 //   * an inelastic event may run only if the stack has room for the
 //     secondary it may create; otherwise the electron waits (PENDING,
 //     gpu_driver.inl:523-528) and retries next step without resampling;
+//   * inelastic events run in batches: their code path (table draws,
+//     energy-loss sampling, the secondary's direction) costs the whole
+//     wavefront however few lanes take it, so a lane whose event is
+//     inelastic waits in the same way until inel_batch lanes want one, one
+//     has waited inel_wait iterations, or no live lane has other work;
 //   * detections update per-pixel SE/BSE counters, a 1-eV histogram and the
 //     work counters used for the roofline.
 // Every wavefront exits once the primary counter is exhausted, its stack is
@@ -66,6 +71,7 @@ struct pers_params {
     uint32_t depth;                     // LDS stack entries per wavefront
     uint32_t ovf;                       // HBM stack entries per wavefront
     uint32_t refill_threshold;
+    uint32_t inel_batch, inel_wait;     // inelastic batching (see above)
     dmlp mlp[2];                        // learned sampler (elastic, inelastic) when LEARNED
     uint32_t learned;                   // bit k: network k replaces table k (LEARNED)
     // explicit primaries (plist != nullptr): primary i is plist[i] with tag ptags[i]
@@ -191,6 +197,7 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
     float dist = 0;
     uint8_t type = 0;
     uint32_t sp = 0;                                // wave-uniform stack size
+    uint32_t waited = 0;                            // iterations an inelastic request has waited (wave-uniform)
     const uint32_t home = blockIdx.x % N_QUEUES;    // this block's XCD
     uint32_t qi = 0;                                // queues found empty so far (wave-uniform)
     // Work counters: per-lane walk counts, and wave-uniform counts of lanes
@@ -276,12 +283,14 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
         // anyway and drops (counts) the secondaries that do not fit, so that
         // every wavefront keeps making progress.
         const bool want = alive && ev == EV_INELASTIC;
-        const uint64_t wm = __ballot(want);
+        const uint64_t wm = __ballot(want), am = __ballot(alive);
+        const bool batch = (uint32_t)__popcll(wm) >= P.inel_batch || waited >= P.inel_wait || wm == am;
+        waited = (wm && !batch) ? waited + 1 : 0;
         const uint32_t room = cap - sp;
-        const bool stuck = room == 0 && wm == __ballot(alive);
-        const bool go = want && (stuck || rank_below(wm) < room);
+        const bool stuck = room == 0 && wm == am;
+        const bool go = want && batch && (stuck || rank_below(wm) < room);
         if (want && !go) pend = true;
-        n_pend += __popcll(__ballot(want && !go));
+        n_pend += __popcll(__ballot(want && batch && !go));
         n_bound += __popcll(__ballot(alive && ev == EV_INTERSECT));
         {
             const uint32_t n_el = __popcll(__ballot(alive && ev == EV_ELASTIC)), n_in = __popcll(__ballot(go));
@@ -431,6 +440,10 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
     uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 128;
     uint32_t thresh = opts && opts->refill_threshold ? opts->refill_threshold : 4;
     uint32_t ovf = opts && opts->overflow_depth ? opts->overflow_depth : 4096;
+    const uint32_t ibatch = opts && opts->inelastic_batch ? opts->inelastic_batch : 8;
+    const uint32_t iwait = opts && opts->inelastic_wait ? opts->inelastic_wait : 8;
+    require(ibatch <= 64, "nbl_driver_run_persistent: inelastic_batch must be 1..64");
+    require(iwait <= 1024, "nbl_driver_run_persistent: inelastic_wait must be <= 1024");
     require(ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
     require(block % 64 == 0 && block >= 64 && block <= 1024, "nbl_driver_run_persistent: block_size must be 64..1024, multiple of 64");
     require(!d->sampler || block <= 768, "nbl_driver_run_persistent: block_size must be <= 768 with a learned sampler");
@@ -496,6 +509,7 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
     if (d->overflow.n < ovf_floats) d->overflow.alloc(ovf_floats);
     P.overflow = d->overflow.p;
     P.depth = depth; P.ovf = ovf; P.refill_threshold = thresh;
+    P.inel_batch = ibatch; P.inel_wait = iwait;
     // energy-deposition tally
     nbl_deposit_grid* dg = out->deposit;
     dbuf<double> ddep, dtot;

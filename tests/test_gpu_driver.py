@@ -234,3 +234,32 @@ def test_persistent_list_mode_and_detection_records():
         bad["dir"][3] = 0
         with pytest.raises(na.NblError):
             drv.run_persistent_list(bad)
+
+
+def test_inelastic_batching_keeps_yields_and_histograms():
+    """Inelastic events wait in batches of 8 by default (nbl_persistent_opts
+    inelastic_batch / inelastic_wait); against no batching (1), the default
+    and a long wait give the same yields and detected-energy histogram
+    within statistics, for every primary."""
+    print("[gpu] inelastic batching", flush=True)
+    tris = scenes.rough_surface(n=128)
+    g = na.Geometry(tris); m = na.Materials([synth.SI])
+    beam = raster_scan(np.zeros(1), np.zeros(1), 20.0, 500.0, 400_000, 0.0, seed=21)
+    res = {}
+    with Driver(g, m, capacity=1024, seed=8) as drv:
+        for name, kw in (("default", {}), ("off", dict(inelastic_batch=1)),
+                         ("long", dict(inelastic_batch=32, inelastic_wait=64))):
+            res[name] = drv.run_persistent(beam, hist=True, **kw)
+    base = res["off"]
+    for name in ("default", "long"):
+        t = res[name]
+        assert t["primaries"] == base["primaries"] and t["stopped"] == 0
+        for k in ("detected_se", "detected_bse"):
+            z = runs.yield_z(base[k], base["primaries"], t[k], t["primaries"])
+            print(f"  {name} {k}: {t[k] / t['primaries']:.4f} vs {base[k] / base['primaries']:.4f}, z {z:+.2f}")
+            assert abs(z) < 4
+        a, b = base["hist"][:600].astype(float), t["hist"][:600].astype(float)
+        keep = (a + b) > 20
+        chi2 = float(np.mean((a - b)[keep] ** 2 / (a + b)[keep]))
+        print(f"  {name} histogram chi2/bin {chi2:.3f} over {keep.sum()} bins")
+        assert chi2 < 1.5, chi2
