@@ -8,6 +8,7 @@ a measured choice of the launch shape and scheduling cadence:
   blocks_per_cu    resident workgroups per compute unit
   stack_depth      LDS entries of each wavefront's secondary stack
   refill_threshold idle lanes needed before a wavefront refills (compaction cadence)
+  inelastic_batch  lanes that must want an inelastic event before a wavefront runs them
 
 Each candidate runs a short sample of the caller's own beam (same geometry,
 materials and energies); the fastest by device time wins.  Results are
@@ -31,7 +32,9 @@ DEFAULT_SPACE = {
     "blocks_per_cu": (2, 4, 8),
     "stack_depth": (64, 128),
     "refill_threshold": (1, 4, 8, 32),
+    "inelastic_batch": (1, 8),
 }
+PATH_MEMO_LEVELS = 16        # per-lane octree path memo in LDS (lds_path_cache::kLevels)
 
 
 def _signature(driver, beam, sample):
@@ -60,7 +63,7 @@ def _sample_beam(beam, sample):
 
 def valid(cfg):
     waves = cfg["block_size"] // 64
-    lds = waves * 9 * cfg["stack_depth"] * 4
+    lds = waves * (9 * cfg["stack_depth"] + PATH_MEMO_LEVELS * 64) * 4
     return lds * cfg["blocks_per_cu"] <= 160 * 1024 and cfg["block_size"] * cfg["blocks_per_cu"] <= 2048
 
 
