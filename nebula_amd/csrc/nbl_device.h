@@ -538,10 +538,19 @@ struct rng_inject {
     __device__ __forceinline__ float unit() { return k < 8 ? u[k++] : 0.5f; }
 };
 
+// log and exp where an error of a few ulp is not amplified (table
+// coordinates log E, the imfp from its logarithm, the exponential draws):
+// v_log_f32 / v_exp_f32 (log2, exp2: 1 ulp for normal arguments) and one
+// multiplication, for normal positive inputs (energies, uniform draws
+// >= 2^-24) and moderate arguments, against the 13-14 instructions of the
+// device library's logf / expf.
+__device__ __forceinline__ float log_fast(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
+__device__ __forceinline__ float exp_fast(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896f); }
+
 // random_generator draws, common/util/random.h:64-83.  uniform_vector takes
 // cos(theta) first, then phi (the reference leaves the order to the compiler).
 template<typename R> __device__ __forceinline__ float rng_phi(R& r) { return 2 * PI_F * r.unit(); }
-template<typename R> __device__ __forceinline__ float rng_exponential(R& r, float tau) { return -tau * logf(r.unit()); }
+template<typename R> __device__ __forceinline__ float rng_exponential(R& r, float tau) { return -tau * log_fast(r.unit()); }
 
 // Square root and normalisation in the physics: one hardware instruction
 // each (v_sqrt_f32, v_rsq_f32, about 1 ulp), as the reference's GPU build
@@ -597,16 +606,16 @@ struct dparticle { f3 pos, dir; float e; };
 template<typename R>
 __device__ __forceinline__ float sample_path(const dmat& m, float kin_energy, R& rng, uint8_t* type)
 {
-    const float le = logf(kin_energy);
+    const float le = log_fast(kin_energy);
     float best = __builtin_inff();
     uint8_t ty = 0;
     {
-        const float imfp = expf(tab1_get(m.el_imfp, le));
+        const float imfp = exp_fast(tab1_get(m.el_imfp, le));
         const float path = rng_exponential(rng, rcp_fast(imfp));
         if (path < best) { best = path; ty = 2; }
     }
     {
-        const float imfp = expf(tab1_get(m.in_imfp, le));
+        const float imfp = exp_fast(tab1_get(m.in_imfp, le));
         const float path = rng_exponential(rng, rcp_fast(imfp));
         if (path < best) { best = path; ty = 1; }
     }
@@ -685,7 +694,7 @@ template<typename R>
 __device__ __forceinline__ bool scatter_begin(const dmat& m, dparticle& p, R& rng, bool elastic, scatter_state& s)
 {
     const float y = rng.unit();
-    const float v = tab2_get(elastic ? m.el_icdf : m.in_icdf, logf(p.e), y);
+    const float v = tab2_get(elastic ? m.el_icdf : m.in_icdf, log_fast(p.e), y);
     return scatter_begin_v(m, p, rng, elastic, v, s);
 }
 

@@ -326,7 +326,7 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
                 // the networks of P.learned (wave-uniform) replace their
                 // tables; a kind left out is read from its table as usual
                 float y = 0.5f, le = 0.0f;
-                if (el || go) { y = rng.unit(); le = logf(p.e); }
+                if (el || go) { y = rng.unit(); le = log_fast(p.e); }
                 float ve = 0, vi = 0;
                 if (P.learned & (1u << MLP_KIND_ELASTIC)) ve = mlp_eval_needed(P.mlp[MLP_KIND_ELASTIC], el, mat, le, y);
                 else if (el) ve = tab2_get(P.mats[mat].el_icdf, le, y);
