@@ -567,7 +567,7 @@ __device__ __forceinline__ f3 normalised_fast(f3 a) { return a * __builtin_amdgc
 // reduction: sin and cos of atan2(y, x) are y/r and x/r with r = |(x, y)|
 // (0 and 1 for r = 0, as atan2(0, 0) = 0), and sin, cos of 2 pi u come from
 // sincospi(2u).  The same vector as the reference's up to rounding.
-__device__ __forceinline__ f3 make_normal_vec_u(f3 dir, float u)
+__device__ __forceinline__ f3 normal_vec_frame(f3 dir, float sin_phi, float cos_phi)
 {
     const float r = sqrt_fast(dir.x * dir.x + dir.y * dir.y);
     float cos_az = 1, sin_az = 0;
@@ -578,9 +578,21 @@ __device__ __forceinline__ f3 make_normal_vec_u(f3 dir, float u)
     }
     const f3 unit_v{ dir.z * cos_az, dir.z * sin_az, -r };
     const f3 unit_u = cross(unit_v, dir);
+    return unit_u * cos_phi + unit_v * sin_phi;
+}
+__device__ __forceinline__ f3 make_normal_vec_u(f3 dir, float u)
+{
     float sin_phi, cos_phi;
     sincospif(2 * u, &sin_phi, &cos_phi);
-    return unit_u * cos_phi + unit_v * sin_phi;
+    return normal_vec_frame(dir, sin_phi, cos_phi);
+}
+// The same with sin and cos of 2 pi u from the hardware (v_sin_f32 and
+// v_cos_f32 take their argument in turns: one instruction each against the
+// 42 of sincospif), for the Kieft scattering frame, whose outputs keep the
+// physics tolerance with it.
+__device__ __forceinline__ f3 make_normal_vec_turn(f3 dir, float u)
+{
+    return normal_vec_frame(dir, __builtin_amdgcn_sinf(u), __builtin_amdgcn_cosf(u));
 }
 // make_unit_vec (vector_math.h:194-206) for phi = 2 pi u.
 __device__ __forceinline__ f3 make_unit_vec_u(float cos_theta, float u)
@@ -702,7 +714,7 @@ template<typename R>
 __device__ __forceinline__ f3 scatter_frame(dparticle& p, R& rng)
 {
     p.dir = normalised_fast(p.dir);
-    return normalised_fast(make_normal_vec_u(p.dir, rng.unit()));
+    return normalised_fast(make_normal_vec_turn(p.dir, rng.unit()));
 }
 
 // elastic.h:88-115 (phonon + recoil loss on); inelastic.h:151-206.
