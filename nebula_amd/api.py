@@ -249,7 +249,7 @@ class Sampler:
         return out
 
     def table(self, kind, material, energy, u):
-        """The same quantity from the material's table (the sampler's judge)."""
+        """The same quantity from the material's table (the reference the sampler is checked against)."""
         return self.materials.table2d(material, self.KINDS[kind], np.log(np.asarray(energy, np.float32)), u)
 
     def ks_per_bin(self, kind, material, edges, n=20000, seed=0):

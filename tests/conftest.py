@@ -9,5 +9,5 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X); runs on the GPU box")
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
     config.addinivalue_line("markers", "slow: longer CPU test")

@@ -3,7 +3,7 @@ single-event physics) against the reference's own outputs (golden fixtures)
 and the CPU oracle.
 
 Bars: triangle ids and table values bit-exact; propagate distances
-bit-exact (the north star allows 2 ulp); physics outputs within the
+bit-exact (no ulp allowance is needed); physics outputs within the
 transcendental tolerance stated in PHYS_RTOL (ROCm device math vs host libm).
 """
 import ctypes as C
