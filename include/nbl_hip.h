@@ -415,7 +415,7 @@ typedef struct {
     uint32_t blocks_per_cu;    /* resident workgroups per CU; 0 = default          */
     uint32_t stack_depth;      /* per-wavefront secondary stack entries in LDS; 0 = default (128) */
     uint32_t refill_threshold; /* refill a wavefront when >= this many lanes idle; 0 = default (4) */
-    uint32_t overflow_depth;   /* further per-wavefront stack entries in HBM; 0 = default (4096) */
+    uint32_t overflow_depth;   /* further per-wavefront stack entries in HBM; 0 = default (16384) */
     uint32_t inelastic_batch;  /* run inelastic events once this many lanes of a wavefront
                                   want one (1 = at once); 0 = default (8)                  */
     uint32_t inelastic_wait;   /* ... or once one has waited this many loop iterations;

@@ -359,6 +359,48 @@ __device__ __forceinline__ void next_cell(uint64_t& location, int axis, f3 dir)
     }
 }
 
+// True when no triangle of the leaf (its leaf_info li, cell half sizes s)
+// can take part in the walk of this lane, so that its candidates need not
+// enter the shared tests; the lane's direction has |d|^2 within 1e-4 of 1,
+// and either
+//   * side cull: the side cone puts d clearly on one side of all the leaf's
+//     normals and the material on that side is the ignored one, so the side
+//     test (octree.inl:128-138) skips every triangle; or
+//   * slab cull: the segment [pos, pos + T d], T = min(exit, to_go) +
+//     EPSILON, lies outside the leaf's slab, so intersect_ray returns no
+//     distance in (0, T] for any of them.  The walk keeps a distance only if
+//     it is in (0, exit + EPSILON], and when to_go <= exit one beyond
+//     to_go + EPSILON cannot change its result (the walk ends without a hit
+//     whenever such a distance would be kept), so dropping the leaf leaves
+//     the walk's result unchanged.
+// slab_culled: the slab cull applied; then all the leaf's candidates pass
+// the side test (|dot(a, d)| > t: the side is decided and is not the
+// ignored one), and the reference tests each of them.
+__device__ __forceinline__ bool leaf_dropped(const leaf_info& li, f3 s, f3 pos, f3 dir, float exit, float to_go,
+    int ignore_mat, bool& slab_culled)
+{
+    slab_culled = false;
+    if (!(fabsf(dot(dir, dir) - 1) < 1e-4f)) return false;
+    const float c = li.ax * dir.x + li.ay * dir.y + li.az * dir.z;
+    const int mat_in = (int)(int16_t)(li.mats & 0xFFFFu), mat_out = (int)(int16_t)(li.mats >> 16);
+    if ((c > li.t && mat_out == ignore_mat) || (c < -li.t && mat_in == ignore_mat)) return true;
+    if (!(fabsf(c) >= li.t + 0.0202f)) return false;
+    const float T = fminf(exit, to_go) + EPSILON;
+    if (!(T <= 2.03f * (s.x + s.y + s.z) + 1e-3f)) return false;
+    const float h0 = li.ax * pos.x + li.ay * pos.y + li.az * pos.z;
+    const float h1 = h0 + T * c;
+    slab_culled = (h0 > li.hi && h1 > li.hi) || (h0 < li.lo && h1 < li.lo);
+    return slab_culled;
+}
+
+// leaf_info k as two 16-B loads
+__device__ __forceinline__ leaf_info load_leaf(const dgeom& g, uint32_t k)
+{
+    const float4* p = reinterpret_cast<const float4*>(g.leaves + k);
+    const float4 a = p[0], b = p[1];
+    return leaf_info{ a.x, a.y, a.z, a.w, b.x, b.y, __float_as_uint(b.z), __float_as_uint(b.w) };
+}
+
 // octree::propagate, geometry/octree.inl:32-192.  Same cell walk (location
 // codes, re-descent from the root, neighbour by bit flip), same AABB and
 // intersection arithmetic and the same "t <= best + EPSILON" replacement
@@ -376,12 +418,20 @@ __device__ __forceinline__ float propagate(const dgeom& g, f3 start, f3 dir, flo
     do {
         f3 c, s;
         const uint64_t child = descend<COUNT>(g, pos, location, c, s, cnt, pc);
-        const uint32_t count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
-        const uint32_t first = count ? g.leaves[(uint32_t)child].first : 0u;
+        uint32_t count = (uint32_t)((child >> 32) & 0x7FFFFFFFu), first = 0;
         int axis;          // -1, -2, -4 for x, y, z; 0 once a triangle is kept
         float best = leaf_exit(c, s, pos, dir, axis);
         uint32_t best_id = NO_TRI;
         if (COUNT) cnt->node_steps += count + 1;
+        if (count > 0) {
+            const leaf_info li = load_leaf(g, (uint32_t)child);
+            first = li.first;
+            bool slab_culled;
+            if (leaf_dropped(li, s, pos, dir, best, to_go, ignore_mat, slab_culled)) {
+                if (COUNT && slab_culled) cnt->tri_tests += count;
+                count = 0;
+            }
+        }
         for (uint32_t k = first; k < first + count; ++k) {
             uint32_t id;
             const float t = leaf_candidate<COUNT>(g.recs, k, pos, dir, ignore_tri, ignore_mat, id, cnt);

@@ -439,7 +439,7 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
     uint32_t per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : (d->sampler ? 3u : 4u);
     uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 128;
     uint32_t thresh = opts && opts->refill_threshold ? opts->refill_threshold : 4;
-    uint32_t ovf = opts && opts->overflow_depth ? opts->overflow_depth : 4096;
+    uint32_t ovf = opts && opts->overflow_depth ? opts->overflow_depth : 16384;
     const uint32_t ibatch = opts && opts->inelastic_batch ? opts->inelastic_batch : 8;
     const uint32_t iwait = opts && opts->inelastic_wait ? opts->inelastic_wait : 8;
     require(ibatch <= 64, "nbl_driver_run_persistent: inelastic_batch must be 1..64");

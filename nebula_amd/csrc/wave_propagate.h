@@ -29,48 +29,6 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& tot
     return x - v;
 }
 
-// True when no triangle of the leaf (its leaf_info li, cell half sizes s)
-// can take part in the walk of this lane, so that its candidates need not
-// enter the shared tests; the lane's direction has |d|^2 within 1e-4 of 1,
-// and either
-//   * side cull: the side cone puts d clearly on one side of all the leaf's
-//     normals and the material on that side is the ignored one, so the side
-//     test (octree.inl:128-138) skips every triangle; or
-//   * slab cull: the segment [pos, pos + T d], T = min(exit, to_go) +
-//     EPSILON, lies outside the leaf's slab, so intersect_ray returns no
-//     distance in (0, T] for any of them.  The walk keeps a distance only if
-//     it is in (0, exit + EPSILON], and when to_go <= exit one beyond
-//     to_go + EPSILON cannot change its result (the walk ends without a hit
-//     whenever such a distance would be kept), so dropping the leaf leaves
-//     the walk's result unchanged.
-// slab_culled: the slab cull applied; then all the leaf's candidates pass
-// the side test (|dot(a, d)| > t: the side is decided and is not the
-// ignored one), and the reference tests each of them.
-__device__ __forceinline__ bool leaf_dropped(const leaf_info& li, f3 s, f3 pos, f3 dir, float exit, float to_go,
-    int ignore_mat, bool& slab_culled)
-{
-    slab_culled = false;
-    if (!(fabsf(dot(dir, dir) - 1) < 1e-4f)) return false;
-    const float c = li.ax * dir.x + li.ay * dir.y + li.az * dir.z;
-    const int mat_in = (int)(int16_t)(li.mats & 0xFFFFu), mat_out = (int)(int16_t)(li.mats >> 16);
-    if ((c > li.t && mat_out == ignore_mat) || (c < -li.t && mat_in == ignore_mat)) return true;
-    if (!(fabsf(c) >= li.t + 0.0202f)) return false;
-    const float T = fminf(exit, to_go) + EPSILON;
-    if (!(T <= 2.03f * (s.x + s.y + s.z) + 1e-3f)) return false;
-    const float h0 = li.ax * pos.x + li.ay * pos.y + li.az * pos.z;
-    const float h1 = h0 + T * c;
-    slab_culled = (h0 > li.hi && h1 > li.hi) || (h0 < li.lo && h1 < li.lo);
-    return slab_culled;
-}
-
-// leaf_info k as two 16-B loads
-__device__ __forceinline__ leaf_info load_leaf(const dgeom& g, uint32_t k)
-{
-    const float4* p = reinterpret_cast<const float4*>(g.leaves + k);
-    const float4 a = p[0], b = p[1];
-    return leaf_info{ a.x, a.y, a.z, a.w, b.x, b.y, __float_as_uint(b.z), __float_as_uint(b.w) };
-}
-
 // A lane's walk between cells (the state octree::propagate keeps across
 // iterations of its cell loop, octree.inl:32-190).
 struct walk_state {
