@@ -263,3 +263,23 @@ def test_inelastic_batching_keeps_yields_and_histograms():
         chi2 = float(np.mean((a - b)[keep] ** 2 / (a + b)[keep]))
         print(f"  {name} histogram chi2/bin {chi2:.3f} over {keep.sum()} bins")
         assert chi2 < 1.5, chi2
+
+
+def test_tiny_stack_waits_for_room_and_counts_drops():
+    """With the smallest stack (64 LDS entries, 1 in HBM) 2 keV cascades run
+    out of room: inelastic events wait for it (pending_retries), a wavefront
+    whose live lanes all wait drops the secondaries that do not fit and
+    counts them as stopped, and every primary still runs to the end."""
+    print("[gpu] tiny stack", flush=True)
+    tris = scenes.cube_slab()
+    g = na.Geometry(tris); m = na.Materials([synth.SI])
+    beam = raster_scan(np.zeros(1), np.zeros(1), 5.0, 2000.0, 20_000, 0.0, seed=31)
+    with Driver(g, m, capacity=1024, seed=12) as drv:
+        t = drv.run_persistent(beam, stack_depth=64, overflow_depth=1)
+        ref = drv.run_persistent(beam)
+    print(f"  tiny: pending {t['pending_retries']} stopped {t['stopped']} secondaries {t['secondaries']}; "
+          f"default: pending {ref['pending_retries']} stopped {ref['stopped']}")
+    assert t["primaries"] == ref["primaries"] == 20_000
+    assert t["pending_retries"] > 0 and ref["stopped"] == 0
+    assert 0 < t["secondaries"] and t["stopped"] < t["secondaries"]
+    assert t["detected_bse"] > 0 and t["detected_se"] > 0
