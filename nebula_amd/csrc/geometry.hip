@@ -13,6 +13,7 @@ struct nbl_geometry_s {
     dbuf<uint64_t> nodes;
     dbuf<tri_rec> recs;
     dbuf<leaf_info> leaves;
+    dbuf<leaf_rope> ropes;
     dbuf<uint32_t> id_rec;
     dgeom view{};
     uint32_t n_in = 0;
@@ -104,6 +105,8 @@ int nbl_geometry_create(const nbl_triangle* tris, uint32_t n, nbl_geometry* out)
         g->recs.upload(g->host.recs.data(), g->host.recs.size());
         g->leaves.alloc(std::max<size_t>(g->host.leaves.size(), 1));
         g->leaves.upload(g->host.leaves.data(), g->host.leaves.size());
+        g->ropes.alloc(std::max<size_t>(g->host.ropes.size(), 6));
+        g->ropes.upload(g->host.ropes.data(), g->host.ropes.size());
         // one record per input id (NO_TRI for triangles dropped as duplicates)
         std::vector<uint32_t> id_rec(n, NO_TRI);
         for (size_t k = g->host.recs.size(); k-- > 0;) id_rec[g->host.recs[k].id] = (uint32_t)k;
@@ -111,7 +114,7 @@ int nbl_geometry_create(const nbl_triangle* tris, uint32_t n, nbl_geometry* out)
         g->id_rec.upload(id_rec.data(), n);
         NBL_HIP(hipDeviceSynchronize());
         dgeom& v = g->view;
-        v.nodes = g->nodes.p; v.recs = g->recs.p; v.leaves = g->leaves.p; v.id_rec = g->id_rec.p;
+        v.nodes = g->nodes.p; v.recs = g->recs.p; v.leaves = g->leaves.p; v.ropes = g->ropes.p; v.n_ropes = (uint32_t)std::max<size_t>(g->host.ropes.size(), 6); v.id_rec = g->id_rec.p;
         v.cx = g->host.center[0]; v.cy = g->host.center[1]; v.cz = g->host.center[2];
         v.hx = g->host.halfsize[0]; v.hy = g->host.halfsize[1]; v.hz = g->host.halfsize[2];
         v.max_extent = g->host.max_extent;
@@ -139,7 +142,7 @@ int nbl_geometry_get_info(nbl_geometry g, nbl_geometry_info* info)
         info->n_leaf_refs = (uint32_t)h.recs.size();
         info->max_depth = h.max_depth;
         info->linearized_size = (uint64_t)info->n_nodes * 8 + h.recs.size() + h.n_leaves;
-        info->device_bytes = g->nodes.bytes() + g->recs.bytes() + g->leaves.bytes();
+        info->device_bytes = g->nodes.bytes() + g->recs.bytes() + g->leaves.bytes() + g->ropes.bytes();
     });
 }
 

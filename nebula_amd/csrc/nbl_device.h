@@ -45,8 +45,8 @@ NBL_HD f3 normalised(f3 a) { const float r = 1 / sqrtf(a.x * a.x + a.y * a.y + a
 //   nodes : 8 x uint64 child words per internal octree node (one 64-B line);
 //           node 0 is the root.  A child word is either an internal node
 //           index (bit 63 clear) or a leaf (bit 63 set) with its record
-//           count in bits [32,63) and, if that is not 0, the index of its
-//           leaf_info (first record, cull bounds) in bits [0,32).
+//           count in bits [32,63) and its ordinal in bits [0,32): the index
+//           of its leaf_info (first record, cull bounds) and of its ropes.
 //   recs  : 48-B triangle records, stored once per leaf reference, in the
 //           leaf's triangle order, so a leaf is one contiguous run.
 // ---------------------------------------------------------------------------
@@ -61,9 +61,9 @@ struct __attribute__((aligned(16))) tri_rec {
 };
 static_assert(sizeof(tri_rec) == 48, "tri_rec is 48 bytes");
 
-// Cull record of a non-empty leaf (leaf_info), in a dense array indexed by
-// the leaf's child word; it also holds the leaf's first record and its
-// materials (every triangle's, when they agree).
+// Cull record of a leaf (leaf_info), in a dense array indexed by the leaf's
+// child word; it also holds the leaf's first record and its materials (every
+// triangle's, when they agree).  Empty leaves have t = 2 and no slab.
 //
 // Side cone (ax, ay, az, t): if every triangle of the leaf has the same
 // materials and the float normals cross(e1, e2) all lie within angle theta
@@ -97,10 +97,22 @@ struct __attribute__((aligned(32))) leaf_info {
 };
 static_assert(sizeof(leaf_info) == 32, "leaf_info is 32 bytes");
 
+// Rope of a leaf face: entry 6 * leaf + 2 * axis + (direction component < 0)
+// holds the child word of the node that the walk reaches by descending the
+// neighbour's location code (next_cell, then octree.inl:58-69) from the
+// root, that node's centre as the descent computes it (the same float sums
+// of +-h), and the code's length; word 0 where the neighbour code leaves the
+// root (the walk ends there and never reads it).
+struct __attribute__((aligned(32))) leaf_rope { uint64_t word; float cx, cy, cz; uint32_t depth; };
+static_assert(sizeof(leaf_rope) == 32, "leaf_rope is 32 bytes");
+constexpr uint32_t NO_ROPE = 0xFFFFFFFFu;
+
 struct dgeom {
     const uint64_t* nodes;
     const tri_rec* recs;
-    const leaf_info* leaves;   // per non-empty leaf, indexed by its child word
+    const leaf_info* leaves;   // per leaf, indexed by its child word
+    const leaf_rope* ropes;    // 6 per leaf
+    uint32_t n_ropes;
     const uint32_t* id_rec;    // input triangle id -> index of one of its records
     float cx, cy, cz;          // root centre
     float hx, hy, hz;          // root half size
@@ -186,7 +198,7 @@ __device__ __forceinline__ unsigned position_octant(f3 pos, f3 a)
 struct no_path_cache {};
 template<bool COUNT>
 __device__ __forceinline__ uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s,
-    prop_counters* cnt, no_path_cache*)
+    prop_counters* cnt, no_path_cache*, uint32_t = NO_ROPE)
 {
     f3 a{ g.cx, g.cy, g.cz }, h{ 0.5f * g.hx, 0.5f * g.hy, 0.5f * g.hz };
     uint32_t node = 0;
@@ -241,49 +253,72 @@ struct lds_path_cache {
         return n;
     }
 };
+// rope entry k as two 16-B loads (k clamped to the table)
+__device__ __forceinline__ leaf_rope load_rope(const dgeom& g, uint32_t k)
+{
+    const float4* p = reinterpret_cast<const float4*>(g.ropes + min(k, g.n_ropes - 1));
+    const float4 a = p[0], b = p[1];
+    const uint64_t w = (uint64_t)__float_as_uint(a.x) | ((uint64_t)__float_as_uint(a.y) << 32);
+    return leaf_rope{ w, a.z, a.w, b.x, __float_as_uint(b.y) };
+}
 template<bool COUNT>
 __device__ __forceinline__ uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s,
-    prop_counters* cnt, lds_path_cache* pc)
+    prop_counters* cnt, lds_path_cache* pc, uint32_t rope = NO_ROPE)
 {
     f3 a{ g.cx, g.cy, g.cz }, h{ 0.5f * g.hx, 0.5f * g.hy, 0.5f * g.hz };
     const uint32_t D = (uint32_t)(63 - __clzll((long long)location)) / 3;   // digits of the code
     const uint32_t M = pc->depth;
-    // leading digits the code shares with the memo's leaf
-    uint32_t common;
-    {
-        const uint32_t md = min(D, M);
-        const uint64_t x = (location >> (3 * (D - md))) ^ (pc->loc >> (3 * (M - md)));
-        common = x == 0 ? md : md - 1 - (uint32_t)(63 - __clzll((long long)x)) / 3;
-    }
-    const bool same = M > 0 && common == D && D == M;
-    if (!same && M > 0 && common == M) common = M - 1;   // a leaf is never inside the code's path
-    uint32_t level = 0;
-    const uint32_t shared = same ? D : common;
-    while (level < shared) {
-        ++level;
-        cell_step((unsigned)(location >> (3 * (D - level))) & 7u, a, h);
-    }
-    if (COUNT) cnt->node_steps += shared;
+    uint32_t level = 0, node = 0;
     uint64_t child = 0;
-    uint32_t node = 0;
     bool leaf = false, matched = false;
-    if (same) {
-        child = pc->leaf;
-        leaf = true;
-    } else if (common == D) {
-        matched = true;                        // the code's node lies on the memo's path
+    if (rope != NO_ROPE) {
+        // The next cell of a walk: the node its code leads to and that node's
+        // centre come from the previous leaf's rope; the memo's levels below
+        // D are that leaf's path, which the neighbour code shares up to its
+        // last digit.  The position part below is shared with new walks.
+        const leaf_rope r = load_rope(g, rope);
+        child = r.word;
+        a = { r.cx, r.cy, r.cz };
+        h = { ldexpf(g.hx, -(int)(D + 1)), ldexpf(g.hy, -(int)(D + 1)), ldexpf(g.hz, -(int)(D + 1)) };
+        level = D;
+        if (COUNT) cnt->node_steps += D;
+        pc->store(level, child);
+        node = (uint32_t)child;
+        leaf = (child & LEAF_BIT) != 0;
     } else {
-        node = pc->node_at(g, common);
-        while (level < D) {
+        // leading digits the code shares with the memo's leaf
+        uint32_t common;
+        {
+            const uint32_t md = min(D, M);
+            const uint64_t x = (location >> (3 * (D - md))) ^ (pc->loc >> (3 * (M - md)));
+            common = x == 0 ? md : md - 1 - (uint32_t)(63 - __clzll((long long)x)) / 3;
+        }
+        const bool same = M > 0 && common == D && D == M;
+        if (!same && M > 0 && common == M) common = M - 1;   // a leaf is never inside the code's path
+        const uint32_t shared = same ? D : common;
+        while (level < shared) {
             ++level;
-            const unsigned oct = (unsigned)(location >> (3 * (D - level))) & 7u;
-            child = g.nodes[(size_t)node * 8 + oct];
-            pc->store(level, child);
-            cell_step(oct, a, h);
-            if (COUNT) ++cnt->node_steps;
-            node = (uint32_t)child;
-            leaf = (child & LEAF_BIT) != 0;
-            if (leaf) break;
+            cell_step((unsigned)(location >> (3 * (D - level))) & 7u, a, h);
+        }
+        if (COUNT) cnt->node_steps += shared;
+        if (same) {
+            child = pc->leaf;
+            leaf = true;
+        } else if (common == D) {
+            matched = true;                        // the code's node lies on the memo's path
+        } else {
+            node = pc->node_at(g, common);
+            while (level < D) {
+                ++level;
+                const unsigned oct = (unsigned)(location >> (3 * (D - level))) & 7u;
+                child = g.nodes[(size_t)node * 8 + oct];
+                pc->store(level, child);
+                cell_step(oct, a, h);
+                if (COUNT) ++cnt->node_steps;
+                node = (uint32_t)child;
+                leaf = (child & LEAF_BIT) != 0;
+                if (leaf) break;
+            }
         }
     }
     while (!leaf) {

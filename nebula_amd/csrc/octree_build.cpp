@@ -248,6 +248,44 @@ leaf_info leaf_bounds(const tri_rec* r, size_t n, const box& bx)
     return res;
 }
 
+// Ropes of every leaf (leaf_rope in nbl_device.h): for each face, the
+// neighbour code of octree::propagate (octree.inl:165-185), descended from the
+// root along its digits (octree.inl:58-69) with the device's centre steps
+// (c += +-h, h halves, in float from the root's centre and half size / 2).
+std::vector<leaf_rope> leaf_ropes(const built_octree& o, const std::vector<uint64_t>& leaf_code, const box& root)
+{
+    std::vector<leaf_rope> ropes(leaf_code.size() * 6, leaf_rope{ 0, 0, 0, 0, 0 });
+    for (size_t i = 0; i < leaf_code.size(); ++i)
+        for (uint32_t a = 0; a < 3; ++a)
+            for (uint32_t neg = 0; neg < 2; ++neg) {
+                const uint64_t mask = 1ull << a, value = neg ? mask : 0;
+                uint64_t loc = leaf_code[i];
+                bool moved = false;
+                while (loc > 1) {
+                    if ((loc & mask) == value) { loc ^= mask; moved = true; break; }
+                    loc >>= 3;
+                }
+                if (!moved) continue;                  // the walk leaves the root here
+                const uint32_t D = (uint32_t)(63 - __builtin_clzll(loc)) / 3;
+                float c[3] = { root.c[0], root.c[1], root.c[2] }, h[3];
+                for (int k = 0; k < 3; ++k) h[k] = 0.5f * root.h[k];
+                uint64_t w = 0;
+                uint32_t node = 0;
+                for (uint32_t l = 1; l <= D; ++l) {
+                    const unsigned oct = (unsigned)(loc >> (3 * (D - l))) & 7u;
+                    w = o.nodes[(size_t)node * 8 + oct];
+                    for (int k = 0; k < 3; ++k) {
+                        c[k] += ((oct >> k) & 1u) ? h[k] : -h[k];
+                        h[k] *= 0.5f;
+                    }
+                    if ((w & LEAF_BIT) && l < D) throw std::runtime_error("neighbour code passes through a leaf");
+                    node = (uint32_t)w;
+                }
+                ropes[i * 6 + 2 * a + neg] = leaf_rope{ w, c[0], c[1], c[2], D };
+            }
+    return ropes;
+}
+
 }  // namespace
 
 built_octree build_octree(const nbl_triangle* in, uint32_t n)
@@ -297,15 +335,19 @@ built_octree build_octree(const nbl_triangle* in, uint32_t n)
     order.reserve(tr.nodes.size());
     order.push_back(0);
     std::vector<box> boxes(tr.nodes.size());          // cell of every node
+    std::vector<uint64_t> code(tr.nodes.size());      // location code of every node
     boxes[0] = root;
+    code[0] = 1;
     for (size_t q = 0; q < order.size(); ++q) {
         const int64_t k = tr.nodes[order[q]].kids;
         if (k >= 0)
             for (int o = 0; o < 8; ++o) {
                 order.push_back(k + o);
                 boxes[k + o] = child_box(boxes[order[q]], o);
+                code[k + o] = (code[order[q]] << 3) | (uint64_t)o;
             }
     }
+    std::vector<uint64_t> leaf_code;
     std::vector<uint64_t> word(tr.nodes.size(), 0);   // child word of every node
     built_octree out;
     uint32_t n_internal = 0;
@@ -317,7 +359,8 @@ built_octree build_octree(const nbl_triangle* in, uint32_t n)
         if (out.recs.size() + b.list.size() > 0xFFFFFFFFull)
             throw std::runtime_error("too many leaf triangle references");
         const size_t start = out.recs.size();
-        word[id] = LEAF_BIT | ((uint64_t)b.list.size() << 32) | (b.list.empty() ? 0ull : (uint64_t)out.leaves.size());
+        word[id] = LEAF_BIT | ((uint64_t)b.list.size() << 32) | (uint64_t)out.leaves.size();
+        leaf_code.push_back(code[id]);
         ++out.n_leaves;
         for (uint32_t t : b.list) {
             const tri_in& s = tr.tris[t];
@@ -330,12 +373,13 @@ built_octree build_octree(const nbl_triangle* in, uint32_t n)
             out.recs.push_back(r);
             if (!seen[t]) { seen[t] = 1; ++out.n_unique; }
         }
+        leaf_info li{ 0, 0, 0, 2, -HUGE_VALF, HUGE_VALF, 0, 0 };
         if (!b.list.empty()) {
-            leaf_info li = leaf_bounds(&out.recs[start], b.list.size(), boxes[id]);
-            li.first = (uint32_t)start;
+            li = leaf_bounds(&out.recs[start], b.list.size(), boxes[id]);
             li.mats = out.recs[start].mats;
-            out.leaves.push_back(li);
         }
+        li.first = (uint32_t)start;
+        out.leaves.push_back(li);
     }
     out.nodes.assign((size_t)n_internal * 8, 0);
     for (int64_t id : order) {
@@ -343,6 +387,7 @@ built_octree build_octree(const nbl_triangle* in, uint32_t n)
         if (b.kids < 0) continue;
         for (int o = 0; o < 8; ++o) out.nodes[word[id] * 8 + o] = word[b.kids + o];
     }
+    out.ropes = leaf_ropes(out, leaf_code, root);
     for (int a = 0; a < 3; ++a) { out.center[a] = root.c[a]; out.halfsize[a] = root.h[a]; }
     // 2*magnitude(halfsize) in its CPU form (octree.inl:263, vector_math.h:117)
     out.max_extent = 2 * std::sqrt(root.h[0] * root.h[0] + root.h[1] * root.h[1] + root.h[2] * root.h[2]);

@@ -12,7 +12,8 @@ namespace nbl {
 struct built_octree {
     std::vector<uint64_t> nodes;     // 8 child words per internal node, BFS order
     std::vector<tri_rec> recs;       // leaf records, BFS leaf order
-    std::vector<leaf_info> leaves;   // per non-empty leaf (index in its child word)
+    std::vector<leaf_info> leaves;   // per leaf, in breadth-first order (index in its child word)
+    std::vector<leaf_rope> ropes;    // 6 per leaf
     float center[3] = { 0, 0, 0 }, halfsize[3] = { 0, 0, 0 };
     float max_extent = 0;
     uint32_t n_unique = 0, n_leaves = 0, max_depth = 0;

@@ -36,11 +36,12 @@ struct walk_state {
     float to_go;         // distance left
     uint64_t location;   // code of the next cell
     uint32_t cells;      // cells visited
+    uint32_t rope;       // rope of the next cell (leaf_rope index), NO_ROPE for the first
 };
 
 __device__ __forceinline__ walk_state walk_begin(f3 start, float distance)
 {
-    return walk_state{ start, distance, 1ull, 0u };
+    return walk_state{ start, distance, 1ull, 0u, NO_ROPE };
 }
 
 // One cell of the walk of every lane with run == true; must be called by all
@@ -53,12 +54,13 @@ __device__ __forceinline__ bool walk_cell(bool run, const dgeom& g, walk_state& 
     uint32_t ignore_tri, int ignore_mat, uint32_t* hit, float* travelled, prop_counters* cnt, CACHE* pc = nullptr)
 {
     const uint32_t lane = lane_id();
-    uint32_t first = 0, count = 0;
+    uint32_t first = 0, count = 0, leaf_id = 0;
     int axis = 0;
     float best = 0;
     if (run) {
         f3 c, s;
-        const uint64_t child = descend<COUNT>(g, w.pos, w.location, c, s, cnt, pc);
+        const uint64_t child = descend<COUNT>(g, w.pos, w.location, c, s, cnt, pc, w.rope);
+        leaf_id = (uint32_t)child;
         count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
         best = leaf_exit(c, s, w.pos, dir, axis);
         if (COUNT) cnt->node_steps += count + 1;
@@ -117,6 +119,11 @@ __device__ __forceinline__ bool walk_cell(bool run, const dgeom& g, walk_state& 
     w.to_go -= best;
     w.pos = w.pos + dir * best;
     next_cell(w.location, axis, dir);
+    {   // the face crossed: axis -1, -2, -4 and the sign rule of next_cell
+        const uint32_t ax = axis == -1 ? 0u : (axis == -2 ? 1u : 2u);
+        const float da = ax == 0 ? dir.x : (ax == 1 ? dir.y : dir.z);
+        w.rope = leaf_id * 6u + 2u * ax + (da >= 0 ? 0u : 1u);
+    }
     if (++w.cells >= MAX_CELLS || w.location <= 1) {
         *hit = NO_TRI; *travelled = distance;
         return true;

@@ -20,12 +20,20 @@
 // decision drops are kept, and every triangle of the leaf is tested with
 // them.
 //
-// usage: leaf_cull_check <triangles.bin> <draws per leaf> <cone scale> <slab scale>
+// It also checks every leaf's ropes (leaf_rope): for each face, the
+// neighbour code by the reference's rule (octree.inl:165-185), descended
+// from the root with the reference's own centre formula (octree.inl:62-67,
+// AABB += AABB.w * halfsize * (bit - 0.5)), must give the rope's child word
+// and, bit for bit, its centre.
+//
+// usage: leaf_cull_check <triangles.bin> <draws per leaf> <cone scale> <slab scale> [swap]
 // A scale below 1 plants a mistake (a smaller side threshold, a thinner
-// slab); the check must then find violations.  Exit status 1 on any.
+// slab), and "swap" one in the ropes (the faces' signs exchanged); the check
+// must then find violations.  Exit status 1 on any.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <random>
 #include <vector>
 
@@ -58,7 +66,8 @@ struct cell { double c[3], h[3]; };
 
 int main(int argc, char** argv)
 {
-    if (argc != 5) { fprintf(stderr, "usage: %s triangles.bin draws cone_scale slab_scale\n", argv[0]); return 2; }
+    if (argc != 5 && argc != 6) { fprintf(stderr, "usage: %s triangles.bin draws cone_scale slab_scale [swap]\n", argv[0]); return 2; }
+    const bool swap_faces = argc == 6;
     FILE* f = fopen(argv[1], "rb");
     if (!f) { perror(argv[1]); return 2; }
     std::vector<nbl_triangle> tris;
@@ -71,24 +80,62 @@ int main(int argc, char** argv)
 
     // leaf cells from the node words (children of a cell: centre +- h/2)
     std::vector<std::pair<uint64_t, cell>> leaves;
+    std::vector<uint64_t> codes;                        // location code of each entry of leaves
     {
-        std::vector<std::pair<uint32_t, cell>> stack;
+        struct item { uint32_t node; cell cl; uint64_t code; };
+        std::vector<item> stack;
         cell root{ { o.center[0], o.center[1], o.center[2] }, { o.halfsize[0], o.halfsize[1], o.halfsize[2] } };
-        stack.push_back({ 0u, root });
+        stack.push_back({ 0u, root, 1ull });
         while (!stack.empty()) {
-            const auto [node, cl] = stack.back();
+            const item it = stack.back();
             stack.pop_back();
             for (int oc = 0; oc < 8; ++oc) {
                 cell ch;
                 for (int a = 0; a < 3; ++a) {
-                    ch.h[a] = 0.5 * cl.h[a];
-                    ch.c[a] = cl.c[a] + (((oc >> a) & 1) ? ch.h[a] : -ch.h[a]);
+                    ch.h[a] = 0.5 * it.cl.h[a];
+                    ch.c[a] = it.cl.c[a] + (((oc >> a) & 1) ? ch.h[a] : -ch.h[a]);
                 }
-                const uint64_t w = o.nodes[(size_t)node * 8 + oc];
-                if (w & LEAF_BIT) leaves.push_back({ w, ch });
-                else stack.push_back({ (uint32_t)w, ch });
+                const uint64_t w = o.nodes[(size_t)it.node * 8 + oc];
+                const uint64_t code = (it.code << 3) | (uint64_t)oc;
+                if (w & LEAF_BIT) { leaves.push_back({ w, ch }); codes.push_back(code); }
+                else stack.push_back({ (uint32_t)w, ch, code });
             }
         }
+    }
+    // ropes
+    size_t n_ropes = 0, rope_bad = 0;
+    for (size_t i = 0; i < leaves.size(); ++i) {
+        const uint32_t ord = (uint32_t)leaves[i].first;
+        for (uint32_t a = 0; a < 3; ++a)
+            for (uint32_t neg = 0; neg < 2; ++neg) {
+                uint64_t loc = codes[i];
+                const unsigned mask = 1u << a, value = neg ? mask : 0u;
+                bool moved = false;
+                while (loc > 1) {
+                    if ((loc & mask) == value) { loc ^= mask; moved = true; break; }
+                    loc >>= 3;
+                }
+                if (!moved) continue;
+                ++n_ropes;
+                int digits = 0;
+                for (uint64_t t = loc; t > 1; t >>= 3) ++digits;
+                float ax = o.center[0], ay = o.center[1], az = o.center[2], aw = 1;
+                uint32_t node = 0;
+                uint64_t w = 0;
+                for (int i2 = 3 * (digits - 1); i2 >= 0; i2 -= 3) {
+                    const unsigned oct = (unsigned)(loc >> i2) & 7u;
+                    w = o.nodes[(size_t)node * 8 + oct];
+                    ax += aw * o.halfsize[0] * ((float)(oct & 1) - 0.5f);
+                    ay += aw * o.halfsize[1] * ((float)((oct & 2) >> 1) - 0.5f);
+                    az += aw * o.halfsize[2] * ((float)((oct & 4) >> 2) - 0.5f);
+                    aw *= 0.5f;
+                    node = (uint32_t)w;
+                }
+                const leaf_rope& r = o.ropes[(size_t)ord * 6 + 2 * a + (swap_faces ? 1 - neg : neg)];
+                const bool same = r.word == w && r.depth == (uint32_t)digits &&
+                                  !memcmp(&r.cx, &ax, 4) && !memcmp(&r.cy, &ay, 4) && !memcmp(&r.cz, &az, 4);
+                if (!same) ++rope_bad;
+            }
     }
 
     std::mt19937_64 rng(1);
@@ -194,6 +241,7 @@ int main(int argc, char** argv)
         }
     }
     printf("candidates %zu in_cone_leaves %zu directions %zu violations %zu in_slab_leaves %zu segments %zu "
-           "slab_violations %zu\n", cand, cand_cone, checks, bad, cand_slab, seg, seg_bad);
-    return (bad != 0 || seg_bad != 0) ? 1 : 0;
+           "slab_violations %zu ropes %zu rope_violations %zu\n", cand, cand_cone, checks, bad, cand_slab, seg, seg_bad,
+           n_ropes, rope_bad);
+    return (bad != 0 || seg_bad != 0 || rope_bad != 0) ? 1 : 0;
 }

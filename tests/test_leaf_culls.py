@@ -1,10 +1,11 @@
 """The octree's leaf culls (nebula_amd/csrc/octree_build.cpp: leaf_bounds;
-wave_propagate.h: leaf_dropped), which let the device walk drop leaves
+nbl_device.h: leaf_dropped), which let the device walk drop leaves
 whose triangles all face the ignored material (side cones) or lie beyond
 the walk's segment (slabs), are checked on the CPU against the reference's
 own float tests (octree.inl:128-138, triangle.inl:7-28) for directions and
-segments at the cull thresholds, on three scenes; planted mistakes in the
-thresholds must be caught.  The device side is covered bit for bit by the
+segments at the cull thresholds, on three scenes; so are the leaves' ropes
+(the node and centre a walk's next cell starts from) against a descent with
+the reference's own centre formula; planted mistakes must be caught.  The device side is covered bit for bit by the
 propagate and transport parity tests."""
 import os
 import shutil
@@ -34,12 +35,12 @@ def checker(tmp_path_factory):
     return d, exe
 
 
-def _run(checker, name, draws, cone_scale=1.0, slab_scale=1.0):
+def _run(checker, name, draws, cone_scale=1.0, slab_scale=1.0, swap=False):
     d, exe = checker
     path = d / f"{name}.bin"
     if not path.exists():
         SCENES[name]().tofile(path)
-    r = subprocess.run([str(exe), str(path), str(draws), str(cone_scale), str(slab_scale)],
+    r = subprocess.run([str(exe), str(path), str(draws), str(cone_scale), str(slab_scale)] + (["swap"] if swap else []),
                        capture_output=True, text=True, timeout=300)
     vals = dict(zip(*[iter(r.stdout.split())] * 2))
     return r.returncode, {k: int(v) for k, v in vals.items()}
@@ -48,7 +49,8 @@ def _run(checker, name, draws, cone_scale=1.0, slab_scale=1.0):
 @pytest.mark.parametrize("name", ["rough", "stripes", "lines"])
 def test_leaf_culls_agree_with_the_reference_tests(checker, name):
     rc, v = _run(checker, name, 24 if name != "lines" else 200)
-    assert rc == 0 and v["violations"] == 0 and v["slab_violations"] == 0, v
+    assert rc == 0 and v["violations"] == 0 and v["slab_violations"] == 0 and v["rope_violations"] == 0, v
+    assert v["ropes"] > 30_000, v
     assert v["directions"] > 100_000 and v["segments"] > 30_000, v
     # the cones and slabs cover most candidates of these scenes (the rest never cull)
     assert v["in_cone_leaves"] > 0.8 * v["candidates"] and v["in_slab_leaves"] > 0.8 * v["candidates"], v
@@ -59,3 +61,5 @@ def test_planted_threshold_mistakes_are_caught(checker):
     assert rc == 1 and v["violations"] > 1000, v
     rc, v = _run(checker, "rough", 8, slab_scale=0.5)
     assert rc == 1 and v["slab_violations"] > 1000, v
+    rc, v = _run(checker, "lines", 1, swap=True)
+    assert rc == 1 and v["rope_violations"] == v["ropes"] > 0, v
