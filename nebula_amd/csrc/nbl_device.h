@@ -285,6 +285,10 @@ __device__ __forceinline__ uint64_t descend(const dgeom& g, f3 pos, uint64_t& lo
         pc->store(level, child);
         node = (uint32_t)child;
         leaf = (child & LEAF_BIT) != 0;
+    } else if (D == 0) {
+        // a new walk (code 1): by position from the root, along the memo
+        // while the octants agree (what the general case below gives for D = 0)
+        matched = true;
     } else {
         // leading digits the code shares with the memo's leaf
         uint32_t common;
