@@ -309,7 +309,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": ach_gbps / HBM_PEAK_GBPS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_persistent", "avg_kernel_ms": avg_kernel_ms, "copy_kernel_GBps": copy_gbps,
-                     "algorithmic_bytes_per_launch": algorithmic_bytes(per_launch)},
+                     "algorithmic_bytes_per_launch": algorithmic_bytes(per_launch),
+                     # the HBM bytes actually moved (profiled) per second, and their share of the peak
+                     "traffic_GBps": traffic / (avg_kernel_ms * 1e-3) / 1e9 if traffic else None,
+                     "traffic_frac": traffic / (avg_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if traffic else None},
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "events_per_s": totals["events"] / wall,
