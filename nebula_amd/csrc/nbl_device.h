@@ -171,18 +171,23 @@ __device__ __forceinline__ float intersect_ray(f3 r0, f3 e1, f3 e2, f3 start, f3
     return t;
 }
 
+// Bit helpers of the descent, usable in host code as well (the descent is
+// checked on the CPU against the reference-order descent).  clz64 needs x != 0.
+NBL_HD int clz64(uint64_t x) { return __builtin_clzll(x); }
+NBL_HD uint32_t as_u32(float f) { return __builtin_bit_cast(uint32_t, f); }
+
 // One level of the descent: the child cell's centre along each axis moves by
 // +-h, and h halves.  The reference's term AABB.w * halfsize * (+-0.5)
 // (octree.inl:64-66, 78-80) is exactly +-h (products by powers of two), and
 // it is added to the centre in the same order, so centres are bit-identical.
-__device__ __forceinline__ void cell_step(unsigned oct, f3& a, f3& h)
+NBL_HD void cell_step(unsigned oct, f3& a, f3& h)
 {
     a.x += (oct & 1u) ? h.x : -h.x;
     a.y += (oct & 2u) ? h.y : -h.y;
     a.z += (oct & 4u) ? h.z : -h.z;
     h.x *= 0.5f; h.y *= 0.5f; h.z *= 0.5f;
 }
-__device__ __forceinline__ unsigned position_octant(f3 pos, f3 a)
+NBL_HD unsigned position_octant(f3 pos, f3 a)
 {
     unsigned oct = 0;
     oct ^= (pos.x > a.x) ? 1u : 0u;
@@ -197,14 +202,14 @@ __device__ __forceinline__ unsigned position_octant(f3 pos, f3 a)
 // child word; c = leaf centre, s = leaf half size (AABB.w * halfsize).
 struct no_path_cache {};
 template<bool COUNT>
-__device__ __forceinline__ uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s,
+NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s,
     prop_counters* cnt, no_path_cache*, uint32_t = NO_ROPE)
 {
     f3 a{ g.cx, g.cy, g.cz }, h{ 0.5f * g.hx, 0.5f * g.hy, 0.5f * g.hz };
     uint32_t node = 0;
     uint64_t child = 0;
     bool leaf = false;
-    for (int i = 60 - __clzll((long long)location); i >= 0; i -= 3) {
+    for (int i = 60 - clz64(location); i >= 0; i -= 3) {
         const unsigned oct = (unsigned)(location >> i) & 7u;
         child = g.nodes[(size_t)node * 8 + oct];
         cell_step(oct, a, h);
@@ -239,13 +244,13 @@ struct lds_path_cache {
     uint64_t loc;        // location code of the memo's leaf (1 = empty)
     uint64_t leaf;       // the leaf's child word
     uint32_t depth;      // the leaf's depth (0 = empty)
-    __device__ __forceinline__ unsigned digit(uint32_t l) const { return (unsigned)(loc >> (3 * (depth - l))) & 7u; }
-    __device__ __forceinline__ void store(uint32_t l, uint64_t child)
+    NBL_HD unsigned digit(uint32_t l) const { return (unsigned)(loc >> (3 * (depth - l))) & 7u; }
+    NBL_HD void store(uint32_t l, uint64_t child)
     {
         if (l < kLevels && !(child & LEAF_BIT)) node[l * 64] = (uint32_t)child;
     }
     // internal node at depth l < depth of the memo's path
-    __device__ __forceinline__ uint32_t node_at(const dgeom& g, uint32_t l) const
+    NBL_HD uint32_t node_at(const dgeom& g, uint32_t l) const
     {
         const uint32_t k = l < kLevels ? l : kLevels - 1;
         uint32_t n = k ? node[k * 64] : 0u;
@@ -254,19 +259,19 @@ struct lds_path_cache {
     }
 };
 // rope entry k as two 16-B loads (k clamped to the table)
-__device__ __forceinline__ leaf_rope load_rope(const dgeom& g, uint32_t k)
+NBL_HD leaf_rope load_rope(const dgeom& g, uint32_t k)
 {
     const float4* p = reinterpret_cast<const float4*>(g.ropes + min(k, g.n_ropes - 1));
     const float4 a = p[0], b = p[1];
-    const uint64_t w = (uint64_t)__float_as_uint(a.x) | ((uint64_t)__float_as_uint(a.y) << 32);
-    return leaf_rope{ w, a.z, a.w, b.x, __float_as_uint(b.y) };
+    const uint64_t w = (uint64_t)as_u32(a.x) | ((uint64_t)as_u32(a.y) << 32);
+    return leaf_rope{ w, a.z, a.w, b.x, as_u32(b.y) };
 }
 template<bool COUNT>
-__device__ __forceinline__ uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s,
+NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s,
     prop_counters* cnt, lds_path_cache* pc, uint32_t rope = NO_ROPE)
 {
     f3 a{ g.cx, g.cy, g.cz }, h{ 0.5f * g.hx, 0.5f * g.hy, 0.5f * g.hz };
-    const uint32_t D = (uint32_t)(63 - __clzll((long long)location)) / 3;   // digits of the code
+    const uint32_t D = (uint32_t)(63 - clz64(location)) / 3;   // digits of the code
     const uint32_t M = pc->depth;
     uint32_t level = 0, node = 0;
     uint64_t child = 0;
@@ -295,7 +300,7 @@ __device__ __forceinline__ uint64_t descend(const dgeom& g, f3 pos, uint64_t& lo
         {
             const uint32_t md = min(D, M);
             const uint64_t x = (location >> (3 * (D - md))) ^ (pc->loc >> (3 * (M - md)));
-            common = x == 0 ? md : md - 1 - (uint32_t)(63 - __clzll((long long)x)) / 3;
+            common = x == 0 ? md : md - 1 - (uint32_t)(63 - clz64(x)) / 3;
         }
         const bool same = M > 0 && common == D && D == M;
         if (!same && M > 0 && common == M) common = M - 1;   // a leaf is never inside the code's path
@@ -325,20 +330,31 @@ __device__ __forceinline__ uint64_t descend(const dgeom& g, f3 pos, uint64_t& lo
             }
         }
     }
+    if (matched) {
+        // Along the memo while the position's octants agree with its digits:
+        // centre arithmetic only.  A matched lane starts above the memo's
+        // leaf (level < M) or with an empty memo (M = 0), so the loop ends
+        // within M - level steps and digit() shifts by at most 3(M-1).
+        while (level < M) {
+            const unsigned oct = position_octant(pos, a);
+            if (pc->digit(level + 1) != oct) break;
+            location = (location << 3) | oct;
+            ++level;
+            if (COUNT) ++cnt->node_steps;
+            cell_step(oct, a, h);
+        }
+        if (M > 0 && level == M) { child = pc->leaf; leaf = true; }
+        else node = pc->node_at(g, level);
+    }
     while (!leaf) {
         const unsigned oct = position_octant(pos, a);
         location = (location << 3) | oct;
         ++level;
         if (COUNT) ++cnt->node_steps;
-        if (matched && level <= M && pc->digit(level) == oct) {
-            if (level == M) { child = pc->leaf; leaf = true; }
-        } else {
-            if (matched) { node = pc->node_at(g, level - 1); matched = false; }
-            child = g.nodes[(size_t)node * 8 + oct];
-            pc->store(level, child);
-            node = (uint32_t)child;
-            leaf = (child & LEAF_BIT) != 0;
-        }
+        child = g.nodes[(size_t)node * 8 + oct];
+        pc->store(level, child);
+        node = (uint32_t)child;
+        leaf = (child & LEAF_BIT) != 0;
         cell_step(oct, a, h);
     }
     pc->loc = location;

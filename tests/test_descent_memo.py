@@ -1,0 +1,89 @@
+"""The persistent kernel's octree descent (nbl_device.h: descend with its
+per-lane path memo and the leaves' ropes) against the reference-order
+descent (descend without a memo: the location code's digits from the root,
+then the position, octree.inl:47-84), on the CPU, call for call: leaf word,
+location code, centre, half size and node-step count must be identical.
+The reference-order descent is the one the scalar propagate kernel uses,
+which the propagate parity tests hold bit for bit to the oracle.  Scenes:
+the rough surface (9 levels) and a cluster of tiny triangles in a large box
+(21 levels, deeper than the memo's LDS levels).  A planted mistake (the
+reference side sees positions on a node's centre plane one ulp higher) must
+be caught, which shows that exact ties are exercised."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from nebula_amd import _lib as L
+from nebula_amd import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HIPCC = shutil.which("hipcc") or ("/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else None)
+CLUSTER = (0.1234, 0.2345, 0.3456)
+
+
+def deep_cluster_scene():
+    """400 triangles of ~1e-6 inside a 2e-5 cube, plus two triangles spanning
+    a 200-wide box: the octree splits down to its depth limit of 21."""
+    rng = np.random.default_rng(5)
+    n = 400
+    t = np.zeros(n + 2, L.TRIANGLE_DTYPE)
+    base = np.array(CLUSTER, np.float32)
+    t["r0"][:n] = base + rng.uniform(-1e-5, 1e-5, (n, 3)).astype(np.float32)
+    t["e1"][:n] = rng.uniform(-1e-6, 1e-6, (n, 3))
+    t["e2"][:n] = rng.uniform(-1e-6, 1e-6, (n, 3))
+    t["r0"][n], t["e1"][n], t["e2"][n] = (-100, -100, -100), (200, 0, 0), (0, 200, 0)
+    t["r0"][n + 1], t["e1"][n + 1], t["e2"][n + 1] = (100, 100, 100), (-200, 0, 0), (0, 0, -200)
+    t["material_in"] = 0
+    t["material_out"] = -123
+    return t
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    if HIPCC is None:
+        pytest.skip("hipcc not available")
+    d = tmp_path_factory.mktemp("descent")
+    exe = d / "descent_memo_check"
+    csrc = os.path.join(ROOT, "nebula_amd", "csrc")
+    r = subprocess.run([HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", "-x", "hip", "--offload-arch=gfx950",
+                        "-I", csrc, "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "descent_memo_check.cpp"), os.path.join(csrc, "octree_build.cpp"),
+                        "-o", str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    scenes_on_disk = {}
+    for name, make in (("rough", scenes.rough_surface), ("deep", deep_cluster_scene)):
+        path = d / f"{name}.tri"
+        np.ascontiguousarray(make()).tofile(path)
+        scenes_on_disk[name] = str(path)
+    return exe, scenes_on_disk
+
+
+def _run(checker, scene, calls, seed, nudge=False):
+    exe, paths = checker
+    args = [str(exe), paths[scene], str(calls), str(seed)]
+    if scene == "deep":
+        args += ["cluster", *(repr(c) for c in CLUSTER), "2e-5"]
+    if nudge:
+        args.append("nudge")
+    return subprocess.run(args, capture_output=True, text=True, timeout=300)
+
+
+@pytest.mark.parametrize("scene,depth", [("rough", 9), ("deep", 21)])
+def test_memo_descent_matches_reference_order(checker, scene, depth):
+    for seed in (1, 2):
+        r = _run(checker, scene, 300_000, seed)
+        assert r.returncode == 0, r.stdout + r.stderr
+        f = r.stdout.split()
+        stats = dict(zip(f[1::2], (int(x) for x in f[2::2])))
+        assert stats["tree_depth"] == depth and stats["max_depth"] == depth
+        assert min(stats["new"], stats["rope"], stats["code"]) > 50_000
+        assert stats["on_plane"] > 30_000
+
+
+@pytest.mark.parametrize("scene", ["rough", "deep"])
+def test_planted_tie_mistake_is_caught(checker, scene):
+    r = _run(checker, scene, 300_000, 3, nudge=True)
+    assert r.returncode == 1 and "MISMATCH" in r.stdout, r.stdout + r.stderr
