@@ -330,19 +330,22 @@ NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s
             }
         }
     }
+    const uint32_t position_from = level;
     if (matched) {
         // Along the memo while the position's octants agree with its digits:
         // centre arithmetic only.  A matched lane starts above the memo's
         // leaf (level < M) or with an empty memo (M = 0), so the loop ends
-        // within M - level steps and digit() shifts by at most 3(M-1).
+        // within M - level steps and the digit shift sh stays in [0, 3(M-1)].
+        // Its code is the memo's code cut to its level, and stays so.
+        uint32_t sh = 3 * (M - level);
         while (level < M) {
             const unsigned oct = position_octant(pos, a);
-            if (pc->digit(level + 1) != oct) break;
-            location = (location << 3) | oct;
+            sh -= 3;
+            if (((unsigned)(pc->loc >> sh) & 7u) != oct) break;
             ++level;
-            if (COUNT) ++cnt->node_steps;
             cell_step(oct, a, h);
         }
+        if (level > position_from) location = pc->loc >> (3 * (M - level));
         if (M > 0 && level == M) { child = pc->leaf; leaf = true; }
         else node = pc->node_at(g, level);
     }
@@ -350,13 +353,13 @@ NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s
         const unsigned oct = position_octant(pos, a);
         location = (location << 3) | oct;
         ++level;
-        if (COUNT) ++cnt->node_steps;
         child = g.nodes[(size_t)node * 8 + oct];
         pc->store(level, child);
         node = (uint32_t)child;
         leaf = (child & LEAF_BIT) != 0;
         cell_step(oct, a, h);
     }
+    if (COUNT) cnt->node_steps += level - position_from;
     pc->loc = location;
     pc->depth = level;
     pc->leaf = child;
