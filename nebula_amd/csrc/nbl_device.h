@@ -245,9 +245,11 @@ struct lds_path_cache {
     uint64_t leaf;       // the leaf's child word
     uint32_t depth;      // the leaf's depth (0 = empty)
     NBL_HD unsigned digit(uint32_t l) const { return (unsigned)(loc >> (3 * (depth - l))) & 7u; }
+    // A leaf's word is stored too: it lands on the level of the memo's own
+    // leaf, which node_at never reads (it reads levels below the leaf).
     NBL_HD void store(uint32_t l, uint64_t child)
     {
-        if (l < kLevels && !(child & LEAF_BIT)) node[l * 64] = (uint32_t)child;
+        if (l < kLevels) node[l * 64] = (uint32_t)child;
     }
     // internal node at depth l < depth of the memo's path
     NBL_HD uint32_t node_at(const dgeom& g, uint32_t l) const
