@@ -406,17 +406,22 @@ __device__ __forceinline__ float leaf_candidate(const tri_rec* __restrict__ recs
 
 // Step to the neighbouring cell across the exit plane (octree.inl:165-185):
 // flip the axis bit at the deepest level where that stays inside the parent.
-__device__ __forceinline__ void next_cell(uint64_t& location, int axis, f3 dir)
+// The reference climbs the code one digit at a time until the digit's axis
+// bit equals `value` (then flips it), or the code is down to 1.  In closed
+// form: the axis bits of the n digits sit at bits axis + 3k (k = 0 is the
+// last digit); the lowest k whose bit allows the move is the lowest set bit
+// of those bits (inverted when value is 0), and the code becomes the code
+// shifted right by 3k with that bit flipped; none such: the code is 1.
+NBL_HD void next_cell(uint64_t& location, int axis, f3 dir)
 {
     const unsigned mask = (unsigned)(-axis);
-    unsigned value;
-    if (mask == 1) value = (dir.x >= 0) ? 0u : 1u;
-    else if (mask == 2) value = (dir.y >= 0) ? 0u : 2u;
-    else value = (dir.z >= 0) ? 0u : 4u;
-    while (location > 1) {
-        if ((location & mask) == value) { location ^= mask; break; }
-        location >>= 3;
-    }
+    const unsigned a = mask == 4u ? 2u : mask >> 1;
+    const bool neg = !((mask == 1u ? dir.x : (mask == 2u ? dir.y : dir.z)) >= 0);   // as the reference's >= 0 test
+    const uint32_t n = location > 1 ? (uint32_t)(63 - clz64(location)) / 3 : 0u;   // digits, <= 21
+    const uint64_t column = 0x1249249249249249ull & ((1ull << (3 * n)) - 1);       // bits 0, 3, .., 3(n-1)
+    const uint64_t bits = location >> a;
+    const uint64_t movable = (neg ? bits : ~bits) & column;
+    location = movable ? (location >> __builtin_ctzll(movable)) ^ mask : 1ull;
 }
 
 // True when no triangle of the leaf (its leaf_info li, cell half sizes s)

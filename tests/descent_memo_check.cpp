@@ -9,7 +9,8 @@
 // as the walk does: new steps (code 1) at positions near the last leaf, far
 // away, or exactly on one of its centre planes; and next cells of a walk (the
 // reference's neighbour code of the last leaf, octree.inl:158-181), with and
-// without the leaf's rope.  Every call must return the same leaf word,
+// without the leaf's rope.  next_cell's closed form is checked against the
+// reference's loop on the walk and on random codes.  Every call must return the same leaf word,
 // location code, centre and half size as the reference-order descent, the
 // same node-step count, and leave the memo's LDS levels on its code's path.
 //
@@ -28,6 +29,46 @@
 #include <vector>
 
 using namespace nbl;
+
+// The reference's neighbour-code step (octree.inl:158-181), as a loop: climb
+// until the digit's axis bit equals the direction's value, flip it.
+static uint64_t ref_next_cell(uint64_t location, int axis, f3 dir)
+{
+    const unsigned mask = (unsigned)(-axis);
+    unsigned value;
+    if (mask == 1) value = (dir.x >= 0) ? 0u : 1u;
+    else if (mask == 2) value = (dir.y >= 0) ? 0u : 2u;
+    else value = (dir.z >= 0) ? 0u : 4u;
+    while (location > 1) {
+        if ((location & mask) == value) { location ^= mask; break; }
+        location >>= 3;
+    }
+    return location;
+}
+
+// next_cell (nbl_device.h, closed form) against the loop on random codes of
+// every length 0..21, all six faces, direction components of both signs,
+// +-0 and NaN.
+static bool check_next_cell(std::mt19937_64& rng, long n)
+{
+    const float comps[] = { 1.f, -1.f, 0.f, -0.f, NAN, 1e-30f, -1e-30f };
+    for (long i = 0; i < n; ++i) {
+        const uint32_t digits = (uint32_t)(rng() % 22);
+        uint64_t code = 1;
+        for (uint32_t k = 0; k < digits; ++k) code = (code << 3) | (rng() & 7u);
+        if (rng() % 4 == 0 && digits) code |= (rng() % 2 ? 0x1249249249249249ull : 0x2492492492492492ull) & ((1ull << (3 * digits)) - 1);
+        const int axis = -(1 << (int)(rng() % 3));
+        const f3 d{ comps[rng() % 7], comps[rng() % 7], comps[rng() % 7] };
+        uint64_t got = code;
+        next_cell(got, axis, d);
+        if (got != ref_next_cell(code, axis, d)) {
+            printf("next_cell MISMATCH code %llx axis %d: %llx / %llx\n", (unsigned long long)code, axis,
+                (unsigned long long)got, (unsigned long long)ref_next_cell(code, axis, d));
+            return false;
+        }
+    }
+    return true;
+}
 
 int main(int argc, char** argv)
 {
@@ -53,6 +94,7 @@ int main(int argc, char** argv)
         } else if (!strcmp(argv[i], "nudge")) nudge = true;
     }
 
+    if (!check_next_cell(rng, 2000000)) return 1;
     const built_octree o = build_octree(tris.data(), (uint32_t)tris.size());
     dgeom g{};
     g.nodes = o.nodes.data();
@@ -91,14 +133,13 @@ int main(int argc, char** argv)
             ++n_new;
         } else {
             const uint32_t a = (uint32_t)(rng() % 3), neg = (uint32_t)(rng() % 2);
-            const uint64_t mask = 1ull << a, value = neg ? mask : 0;
-            loc = last_loc;
-            bool moved = false;
-            while (loc > 1) {
-                if ((loc & mask) == value) { loc ^= mask; moved = true; break; }
-                loc >>= 3;
-            }
-            if (!moved) { last_loc = 1; --it; continue; }     // the walk leaves the root: no next cell
+            const f3 d{ a == 0 ? (neg ? -0.6f : 0.6f) : 0.5f, a == 1 ? (neg ? -0.6f : 0.6f) : 0.5f,
+                        a == 2 ? (neg ? -0.6f : 0.6f) : 0.5f };
+            loc = ref_next_cell(last_loc, -(1 << a), d);
+            uint64_t closed = last_loc;
+            next_cell(closed, -(1 << a), d);
+            if (closed != loc) { printf("next_cell MISMATCH on the walk at call %ld\n", it); return 1; }
+            if (loc <= 1) { last_loc = 1; --it; continue; }   // the walk leaves the root: no next cell
             const float half = a == 0 ? ls.x : a == 1 ? ls.y : ls.z;
             const float step = (neg ? -1.f : 1.f) * half * (0.5f + 0.6f * (U(rng) + 1.f));
             if (a == 0) pos.x += step; else if (a == 1) pos.y += step; else pos.z += step;

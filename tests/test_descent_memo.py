@@ -8,7 +8,9 @@ which the propagate parity tests hold bit for bit to the oracle.  Scenes:
 the rough surface (9 levels) and a cluster of tiny triangles in a large box
 (21 levels, deeper than the memo's LDS levels).  A planted mistake (the
 reference side sees positions on a node's centre plane one ulp higher) must
-be caught, which shows that exact ties are exercised."""
+be caught, which shows that exact ties are exercised.  The neighbour-code
+step (next_cell, in closed form) is checked against the reference's
+digit-by-digit loop on every walk step and on random codes."""
 import os
 import shutil
 import subprocess
