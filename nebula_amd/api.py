@@ -120,6 +120,14 @@ class PreparedMaterial:
             call("nbl_penn_prepare", C.byref(self.penn_struct), ptr(self.in_imfp), ptr(self.in_w0),
                  ptr(self.penn_q), ptr(self.penn_ion), C.byref(self.desc))
 
+    @classmethod
+    def from_desc(cls, desc, keep=None, name=""):
+        """A material whose tables are already prepared: desc points at
+        arrays that keep owns (they must outlive every Materials made from it)."""
+        self = cls.__new__(cls)
+        self.spec, self.name, self._raw, self.desc = None, name, keep, desc
+        return self
+
     @property
     def inelastic(self) -> str:
         return "penn" if self.desc.inelastic_model == L.INELASTIC_PENN else "kieft"

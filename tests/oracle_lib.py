@@ -1,6 +1,10 @@
 """ctypes bindings of the CPU oracle (oracle/_build/liborc.so) and of the
-reference-code harness (oracle/_ref/libnbl_ref.so, present only where the
-reference sources were available at build time).  Test infrastructure."""
+reference-code harness (oracle/_ref/libnbl_ref.so).  Test infrastructure.
+
+The harness is compiled from the reference's own sources, so it is loaded
+only where those sources are (REF_SOURCES): on a machine without them the
+GPU tests check against the committed fixtures and the oracle alone, and a
+stray copy of the library is never loaded."""
 from __future__ import annotations
 
 import ctypes as C
@@ -11,6 +15,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORC_PATH = os.path.join(ROOT, "oracle", "_build", "liborc.so")
 REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libnbl_ref.so")
+REF_SOURCES = "/root/reference/source/geometry/octree.h"
 
 _P, _U32, _I32, _U64, _F, _D = C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64, C.c_float, C.c_double
 
@@ -22,10 +27,12 @@ class RunStats(C.Structure):
                 ("secondaries", _U64), ("seconds", _D), ("threads", _U32)]
 
 
-def _load_orc():
-    if not os.path.exists(ORC_PATH):
-        raise ImportError(f"oracle not built: {ORC_PATH} (run make -C oracle)")
-    lib = C.CDLL(ORC_PATH)
+def load_oracle(path=ORC_PATH):
+    """The oracle library at path (liborc.so, or a test's mutated build of
+    its source) with the argument types of nbl_oracle.h set."""
+    if not os.path.exists(path):
+        raise ImportError(f"oracle not built: {path} (run make -C oracle)")
+    lib = C.CDLL(path)
     sig = {
         "orc_geometry_create": (_P, [_P, _U32]),
         "orc_geometry_destroy": (None, [_P]),
@@ -67,7 +74,7 @@ def _load_orc():
 
 
 def _load_ref():
-    if not os.path.exists(REF_PATH):
+    if not (os.path.exists(REF_PATH) and os.path.exists(REF_SOURCES)):
         return None
     lib = C.CDLL(REF_PATH)
     sig = {
@@ -96,7 +103,7 @@ def _load_ref():
     return lib
 
 
-orc = _load_orc()
+orc = load_oracle()
 ref = _load_ref()
 
 

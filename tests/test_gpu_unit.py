@@ -13,9 +13,11 @@ import numpy as np
 import pytest
 
 import oracle_lib as ol
+import physics_pins as pp
 import nebula_amd as na
 from nebula_amd import _lib as L
 from nebula_amd import scenes, synth
+from nebula_amd.api import PreparedMaterial
 
 pytestmark = pytest.mark.gpu
 
@@ -229,13 +231,19 @@ def test_bulk_physics_matches_oracle(event):
         print(f"  secondary within tol: gpu {sok.mean():.6f}, oracle+1ulp {sok_ulp.mean():.6f}")
         assert sok.mean() > 0.9995 or (sok.mean() > 0.99 and sok.mean() >= sok_ulp.mean())
         # energy bookkeeping (inelastic.h:187, 194): E_out + E_SE = E_in - B + E_F,
-        # i.e. the loss omega is shared exactly; check on the device outputs
-        e_in = parts["kin_energy"].astype(np.float64)[both]
-        e_out = g["particles"]["kin_energy"].astype(np.float64)[both]
-        e_se = g["secondary"]["kin_energy"].astype(np.float64)[both]
-        spec_f = np.array([s.fermi for s in specs])[mat[both]]
-        binding = e_in - e_out + spec_f - e_se            # = B, up to float32 rounding of E
-        assert np.all(binding > -1e-5 * e_in - 1e-5) and np.all(binding < 1839.0 + 1e-5 * e_in + 1)
+        # where B is one of the material's own binding energies: a value of
+        # its prepared binding table, its band gap, or 0 (inelastic.h:131-151)
+        e_in = parts["kin_energy"].astype(np.float64)
+        e_out = g["particles"]["kin_energy"].astype(np.float64)
+        e_se = g["secondary"]["kin_energy"].astype(np.float64)
+        binding = e_in - e_out + np.array([d.fermi for d in m.descs])[mat] - e_se
+        tol = 4 * np.spacing(parts["kin_energy"]).astype(np.float64)
+        for k, prep in enumerate(m.prepared):
+            sel = both & (mat == k)
+            allowed = np.unique(np.concatenate([prep.ion[prep.ion >= 0].ravel(), [0.0],
+                                                [prep.desc.band_gap] if prep.desc.band_gap >= 0 else []]))
+            gap = np.min(np.abs(binding[sel][:, None] - allowed[None, :].astype(np.float32).astype(np.float64)), axis=1)
+            assert sel.sum() > 1000 and np.all(gap <= tol[sel]), (k, gap.max())
 
 
 def test_boundary_physics_matches_oracle():
@@ -260,9 +268,48 @@ def test_boundary_physics_matches_oracle():
     for k in ("pos", "dir", "kin_energy"):
         assert np.array_equal(gg["particles"][k].view(np.uint32), oo["particles"][k].view(np.uint32))
     assert (gg["status"] == L.OUT_DETECTED).sum() > 0
-    # special cases (boundary_intersect.h:78-101): a MIRROR reflects with |d| kept
-    # and E unchanged; transmission probability T = 4s/(1+s)^2 lies in [0, 1]
-    out_side = np.where(np.einsum("ij,ij->i", np.cross(tris["e1"][last], tris["e2"][last]),
-                                  parts["dir"]) > 0, tris["material_out"][last], tris["material_in"][last])
-    alive = (gg["status"] == L.OUT_ALIVE) & (out_side >= -123)
-    assert np.all(gg["particles"]["kin_energy"][alive] > 0)
+
+
+def _pin_device():
+    pm = pp.PinMaterials()
+    tris = pp.pin_triangles()
+    m = na.Materials([PreparedMaterial.from_desc(d, pm) for d in pm.descs])
+    g = na.Geometry(tris)
+    run = lambda event, parts, mat, u, last_tri=None: m.physics(event, parts, mat, u, last_tri=last_tri, geometry=g)
+    return pm, tris, run, m, g
+
+
+@pytest.mark.parametrize("check", list(pp.CHECKS))
+def test_device_physics_meets_closed_forms(check):
+    """The closed-form pins of tests/physics_pins.py on the device's events:
+    every special material of boundary_intersect.h:78-101 (detector, LT50 and
+    GE50 at and around 50 eV, terminator, mirror, NOP), transmission probed
+    just below and above T, refraction, the Kieft elastic and inelastic
+    closed forms and the free-path tie rule."""
+    print(f"[gpu] closed-form physics {check}", flush=True)
+    pm, tris, run, m, g = _pin_device()
+    pp.CHECKS[check](run, pm, tris)
+
+
+def test_special_material_boundaries_bit_exact():
+    """Boundary crossings on triangles of every special material, both
+    orientations: the device equals the oracle bit for bit (+ - * / sqrt only)."""
+    print("[gpu] special-material boundaries", flush=True)
+    pm, tris, run, m, g = _pin_device()
+    orun = pp.OracleRunner(ol.orc, pm, tris)
+    rng = np.random.default_rng(23)
+    n = 100_000
+    parts = random_particles(n, rng, emin=1.0, emax=300.0)
+    parts["kin_energy"][::9] = 50.0
+    last = rng.integers(0, len(tris), n).astype(np.uint32)
+    t = tris[last]
+    side = np.einsum("ij,ij->i", np.cross(t["e1"], t["e2"]), parts["dir"]) > 0
+    mat = np.where(side, t["material_in"], t["material_out"]).astype(np.int32)
+    mat[mat < -123] = 0                              # a particle never sits in a special material
+    u = rng.uniform(0, 1, (n, L.NUM_UNIFORMS)).astype(np.float32)
+    a, b = run(L.EVENT_INTERSECT, parts, mat, u, last_tri=last), orun(L.EVENT_INTERSECT, parts, mat, u, last_tri=last)
+    assert np.array_equal(a["status"], b["status"]) and np.array_equal(a["material"], b["material"])
+    for k in ("pos", "dir", "kin_energy"):
+        assert np.array_equal(a["particles"][k].view(np.uint32), b["particles"][k].view(np.uint32)), k
+    for code in (L.OUT_DETECTED, L.OUT_TERMINATED, L.OUT_ALIVE):
+        assert (a["status"] == code).sum() > 1000, code
