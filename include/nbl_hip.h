@@ -401,7 +401,8 @@ typedef struct {
     uint64_t lookups_1d, lookups_2d, rounddowns;
     uint64_t node_steps, tri_tests, boundary_events, secondaries, pending_retries;
     uint64_t stopped;        /* electrons dropped: 2^20-step cap reached, or a
-                                secondary with no stack space left (normally 0) */
+                                secondary with no stack space left (normally 0;
+                                nonzero is an error unless opts->allow_drops) */
     double   kernel_ms;      /* device time of the transport launch(es)            */
     uint32_t* pixel_se;      /* optional [n_pixels] out (may be NULL)              */
     uint32_t* pixel_bse;     /* optional [n_pixels] out (may be NULL)              */
@@ -420,11 +421,20 @@ typedef struct {
                                   want one (1 = at once); 0 = default (8)                  */
     uint32_t inelastic_wait;   /* ... or once one has waited this many loop iterations;
                                   0 = default (8)                                          */
-    uint32_t reserved[1];
+    uint32_t allow_drops;      /* 0 (default): a run that stopped any electron (a
+                                  secondary with no stack room left, or the 2^20-step
+                                  cap) returns NBL_ERR_RUNTIME, tallies filled but
+                                  biased; 1: such a run returns NBL_OK and counts them
+                                  in nbl_tallies.stopped                                   */
 } nbl_persistent_opts;
 
 /* Runs every primary of the batch to the end of its cascade in one
- * persistent launch; per-pixel SE/BSE tallies and counters go to *out. */
+ * persistent launch; per-pixel SE/BSE tallies and counters go to *out.
+ * An inelastic event waits (PENDING, gpu_driver.inl:523-528) while its
+ * wavefront's secondary stack is full; only a wavefront whose 64 lanes all
+ * wait at once goes ahead and drops what does not fit.  Unless
+ * opts->allow_drops, a run that dropped or step-capped any electron returns
+ * NBL_ERR_RUNTIME (with *out filled) instead of silently biased tallies. */
 int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch,
     const nbl_persistent_opts* opts, nbl_tallies* out);
 /* The same transport over an explicit list of n primaries with tags (the

@@ -209,9 +209,11 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
 
     for (;;) {
         // ---- refill idle lanes: stack first, then fresh primaries ----------
+        // (a full stack refills at once: popping is what makes room for
+        // the secondaries of waiting inelastic events)
         const uint64_t idle = __ballot(!alive);
         const uint32_t n_idle = __popcll(idle);
-        if (n_idle && (n_idle >= P.refill_threshold || n_idle == 64)) {
+        if (n_idle && (n_idle >= P.refill_threshold || n_idle == 64 || sp == cap)) {
             const uint32_t take = min(n_idle, sp);
             const uint32_t r = rank_below(idle);
             if (!alive && r < take) {
@@ -279,9 +281,10 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
             }
         }
         // ---- inelastic events need room for a secondary --------------------
-        // A wavefront whose live lanes all wait on a full stack goes ahead
+        // A wavefront whose live lanes all wait on a full stack (no lane is
+        // idle: an idle lane would have popped an entry above) goes ahead
         // anyway and drops (counts) the secondaries that do not fit, so that
-        // every wavefront keeps making progress.
+        // every wavefront keeps making progress; the host reports the drops.
         const bool want = alive && ev == EV_INELASTIC;
         const uint64_t wm = __ballot(want), am = __ballot(alive);
         const bool batch = (uint32_t)__popcll(wm) >= P.inel_batch || waited >= P.inel_wait || wm == am;
@@ -585,6 +588,10 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
     out->stopped = cnt[C_KILLED];
     if (dg) dg->count = cnt[C_DEPN];
     out->kernel_ms = ms;
+    if (out->stopped && !(opts && opts->allow_drops))
+        throw std::runtime_error("nbl_driver_run_persistent: " + std::to_string(out->stopped) +
+            " electrons stopped (secondaries dropped for lack of stack room, or the 2^20-step cap); the tallies "
+            "are biased. Raise overflow_depth, or set allow_drops to accept them");
 }
 
 }  // namespace

@@ -159,24 +159,27 @@ class Driver:
 
     def run_persistent(self, beam, block_size=0, blocks_per_cu=0, stack_depth=0, refill_threshold=0,
                        per_pixel=False, hist=False, deposit=None, detected=0, inelastic_batch=0, inelastic_wait=0,
-                       overflow_depth=0):
+                       overflow_depth=0, allow_drops=False):
         """One persistent launch over the pixel batch `beam`.  deposit =
         dict(shape=(nx, ny, nz), lo=(x, y, z), hi=(x, y, z)) turns on the
         energy-deposition tally: out["deposit"] is the [nz, ny, nx] grid (eV),
         out["deposit_total"] and out["deposit_count"] cover every deposit.
         detected = capacity of the detected-electron record buffer (0: off):
         out["detected"] = (particles, tags), out["detected_count"].
-        inelastic_batch, inelastic_wait, overflow_depth: nbl_persistent_opts (0 = defaults)."""
+        inelastic_batch, inelastic_wait, overflow_depth: nbl_persistent_opts (0 = defaults).
+        A run that stops any electron (a secondary with no stack room left, or
+        the step cap) raises NblError unless allow_drops; then out["stopped"]
+        counts them."""
         px = np.ascontiguousarray(beam["pixels"], L.PIXEL_DTYPE)
         d = beam.get("dir", (0.0, 0.0, -1.0))
         b = L.PixelBatch(pixels=ptr(px), n_pixels=len(px), dir=L.Vec3(*d), seed=int(beam.get("seed", 0)))
         return self._persistent("nbl_driver_run_persistent", (C.byref(b),), len(px), block_size, blocks_per_cu,
                                 stack_depth, refill_threshold, per_pixel, hist, deposit, detected,
-                                inelastic_batch, inelastic_wait, overflow_depth)
+                                inelastic_batch, inelastic_wait, overflow_depth, allow_drops)
 
     def run_persistent_list(self, primaries, tags=None, block_size=0, blocks_per_cu=0, stack_depth=0,
                             refill_threshold=0, hist=False, deposit=None, detected=0, inelastic_batch=0,
-                            inelastic_wait=0, overflow_depth=0):
+                            inelastic_wait=0, overflow_depth=0, allow_drops=False):
         """The persistent transport over explicit primaries (PARTICLE_DTYPE),
         tags default to their indices; per-pixel tallies are not available."""
         prim = np.ascontiguousarray(primaries, L.PARTICLE_DTYPE)
@@ -184,13 +187,15 @@ class Driver:
         self._keep_list = (prim, tags)
         return self._persistent("nbl_driver_run_persistent_list", (ptr(prim), ptr(tags), len(prim)), 0, block_size,
                                 blocks_per_cu, stack_depth, refill_threshold, False, hist, deposit, detected,
-                                inelastic_batch, inelastic_wait, overflow_depth)
+                                inelastic_batch, inelastic_wait, overflow_depth, allow_drops)
 
     def _persistent(self, fn, head, n_px, block_size, blocks_per_cu, stack_depth, refill_threshold, per_pixel,
-                    hist, deposit, detected, inelastic_batch=0, inelastic_wait=0, overflow_depth=0):
+                    hist, deposit, detected, inelastic_batch=0, inelastic_wait=0, overflow_depth=0,
+                    allow_drops=False):
         o = L.PersistentOpts(block_size=block_size, blocks_per_cu=blocks_per_cu, stack_depth=stack_depth,
                              refill_threshold=refill_threshold, inelastic_batch=inelastic_batch,
-                             inelastic_wait=inelastic_wait, overflow_depth=overflow_depth)
+                             inelastic_wait=inelastic_wait, overflow_depth=overflow_depth,
+                             allow_drops=int(bool(allow_drops)))
         t = L.Tallies()
         se = np.zeros(n_px, np.uint32) if per_pixel else None
         bse = np.zeros(n_px, np.uint32) if per_pixel else None

@@ -268,14 +268,18 @@ def test_inelastic_batching_keeps_yields_and_histograms():
 def test_tiny_stack_waits_for_room_and_counts_drops():
     """With the smallest stack (64 LDS entries, 1 in HBM) 2 keV cascades run
     out of room: inelastic events wait for it (pending_retries), a wavefront
-    whose live lanes all wait drops the secondaries that do not fit and
-    counts them as stopped, and every primary still runs to the end."""
+    whose 64 lanes all wait drops the secondaries that do not fit and counts
+    them as stopped, and every primary still runs to the end.  Dropping is
+    an error (NBL_ERR_RUNTIME naming the count) unless the caller allows it."""
     print("[gpu] tiny stack", flush=True)
     tris = scenes.cube_slab()
     g = na.Geometry(tris); m = na.Materials([synth.SI])
     beam = raster_scan(np.zeros(1), np.zeros(1), 5.0, 2000.0, 20_000, 0.0, seed=31)
     with Driver(g, m, capacity=1024, seed=12) as drv:
-        t = drv.run_persistent(beam, stack_depth=64, overflow_depth=1)
+        with pytest.raises(L.NblError) as e:
+            drv.run_persistent(beam, stack_depth=64, overflow_depth=1)
+        assert e.value.code == L.NBL_ERR_RUNTIME and "electrons stopped" in str(e.value)
+        t = drv.run_persistent(beam, stack_depth=64, overflow_depth=1, allow_drops=True)
         ref = drv.run_persistent(beam)
     print(f"  tiny: pending {t['pending_retries']} stopped {t['stopped']} secondaries {t['secondaries']}; "
           f"default: pending {ref['pending_retries']} stopped {ref['stopped']}")
