@@ -33,7 +33,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E, AMD public spec
-# algorithmic bytes (one event = one init + event step of one electron)
+N_SIMDS = 1024           # 256 CUs x 4 SIMDs
+# algorithmic bytes per unit of work (one event = one init + event step of one
+# electron); the kernel's counters count the work it does: node_steps = child
+# words, leaf records and rope records it loads, tri_tests = triangle records
+# it tests (leaves a cull drops cost none)
 BYTES = dict(event=122, lookup_1d=8, lookup_2d=16, rounddown=4, node_step=4, tri_test=44,
              boundary=52, secondary=65)
 
@@ -51,11 +55,11 @@ WORKLOAD_TEXT = {
 }
 
 
-def profiled_traffic(workload, primaries, kernel_ms):
-    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
-    copy-kernel calibrated) for this workload, from the committed profile
-    summary; None unless the workload and launch size match and this run's
-    kernel time is within 15 % of the profiled one (same kernel build)."""
+def profiled(workload, primaries, kernel_ms):
+    """The committed rocprofv3 figures of this workload's kernel (HBM bytes
+    per launch from FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated; issue
+    counters), or None unless the workload and launch size match and this
+    run's kernel time is within 15 % of the profiled one (same kernel build)."""
     summary = {"rough": "k_persistent_rough_surface_300eV"}.get(workload)
     if summary is None:
         return None, None
@@ -68,7 +72,22 @@ def profiled_traffic(workload, primaries, kernel_ms):
         return None, None
     if not abs(kernel_ms / prof["avg_kernel_ms"] - 1) < 0.15:
         return None, None
-    return float(prof["traffic_bytes"]), os.path.relpath(path, ROOT)
+    return prof, os.path.relpath(path, ROOT)
+
+
+def host_cpu():
+    """Model name and logical CPU count of this host, and the threads this
+    process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
 
 
 def algorithmic_bytes(t):
@@ -192,8 +211,12 @@ def main():
     ap.add_argument("--workload", default="rough",
                     help="rough (default: the headline config), stripes8, rough500, lines, scan3, cube")
     ap.add_argument("--primaries", type=int, default=0, help="primaries per step (0 = the workload's own)")
-    ap.add_argument("--cpu-sample", type=int, default=600_000)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="primaries in the CPU baseline's sample (0 = 40,000 per host thread, at least 600,000)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0 = every thread this process may run on, "
+                         "as cpu_mt_main.cpp:197 takes hardware_concurrency())")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-table sampler comparison")
     ap.add_argument("--no-penn", action="store_true", help="skip the full Penn run of the same workload")
@@ -264,21 +287,35 @@ def main():
     avg_kernel_ms = float(np.mean(kernel_ms))
     per_launch = {k: v / args.steps for k, v in totals.items()}
     ach_gbps = algorithmic_bytes(per_launch) / (avg_kernel_ms * 1e-3) / 1e9
-    traffic, traffic_src = profiled_traffic(args.workload, n_step, avg_kernel_ms)
+    prof, prof_src = profiled(args.workload, n_step, avg_kernel_ms)
+    traffic = float(prof["traffic_bytes"]) if prof else None
+    issue = None
+    if prof and prof.get("valu_insts") and prof.get("gpu_cycles"):
+        # every VALU instruction occupies its SIMD for 4 cycles (64 lanes on 16-wide units)
+        busy = 4 * prof["valu_insts"] / N_SIMDS
+        issue = {"what": "vector ALU issue", "valu_insts_per_launch": prof["valu_insts"],
+                 "valu_busy_cycles_per_simd": busy, "launch_cycles": prof["gpu_cycles"],
+                 "frac": busy / prof["gpu_cycles"], "valu_insts_per_event": prof["valu_insts"] / per_launch["events"],
+                 "source": prof_src}
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        st = cpu_baseline(tris, specs, base_beam, args.cpu_sample, args.cpu_seconds, threads)
+        hc = host_cpu()
+        threads = args.cpu_threads or hc["affinity"]
+        sample = args.cpu_sample or max(600_000, 40_000 * threads)
+        st = cpu_baseline(tris, specs, base_beam, sample, args.cpu_seconds, threads)
         cpu = {"value": st["primaries"] / st["seconds"], "unit": "primaries/s", "cores": int(st["threads"]),
                "kind": "port",
-               "sample": f"oracle cpu_mt restatement, {st['primaries']} primaries drawn from the same pixel "
-                         f"batch (sample of {args.cpu_sample}, stopped at {args.cpu_seconds:.0f} s), "
+               "sample": f"oracle cpu_mt restatement on {st['threads']} threads (every thread of this process's "
+                         f"affinity, as cpu_mt_main.cpp:197), {st['primaries']} primaries drawn from the same pixel "
+                         f"batch (sample of {sample}, stopped at {args.cpu_seconds:.0f} s), "
                          f"{st['seconds']:.1f} s, {st['events'] / st['seconds']:.3g} events/s",
+               "host_nproc": hc["nproc"], "host_affinity_threads": hc["affinity"], "cpu_model": hc["model"],
                "events_per_s": st["events"] / st["seconds"],
                "se_yield": st["detected_se"] / max(st["primaries"], 1),
                "bse_yield": st["detected_bse"] / max(st["primaries"], 1)}
-        print(f"cpu baseline: {cpu['value']:.0f} primaries/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
+        print(f"cpu baseline: {cpu['value']:.0f} primaries/s on {cpu['cores']} threads "
+              f"({hc['model']}, nproc {hc['nproc']})", file=sys.stderr, flush=True)
 
     penn = None
     if not args.no_penn and all(s.inelastic == "kieft" for s in specs):
@@ -306,13 +343,18 @@ def main():
                    "materials": len(specs), "pixels": int(len(base_beam["pixels"])),
                    "sampler": "table", "kernel": "persistent",
                    "launch": {k: v for k, v in opts.items()}},
+        # HBM roofline of the persistent kernel: algorithmic bytes of the work
+        # the launch did (counters x bytes per unit) per second of launch; the
+        # HBM bytes it actually moved (profiled) beside it; and the bound the
+        # profile measures (vector issue), with its own fraction
         "roofline": {"bound": "hbm", "achieved": ach_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": ach_gbps / HBM_PEAK_GBPS, "traffic": traffic, "traffic_source": traffic_src,
+                     "frac": ach_gbps / HBM_PEAK_GBPS, "traffic": traffic, "traffic_source": prof_src,
                      "kernel": "k_persistent", "avg_kernel_ms": avg_kernel_ms, "copy_kernel_GBps": copy_gbps,
                      "algorithmic_bytes_per_launch": algorithmic_bytes(per_launch),
-                     # the HBM bytes actually moved (profiled) per second, and their share of the peak
+                     "bytes_per_unit": BYTES,
                      "traffic_GBps": traffic / (avg_kernel_ms * 1e-3) / 1e9 if traffic else None,
-                     "traffic_frac": traffic / (avg_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if traffic else None},
+                     "traffic_frac": traffic / (avg_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if traffic else None,
+                     "measured_bound": issue},
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "events_per_s": totals["events"] / wall,

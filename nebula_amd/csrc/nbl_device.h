@@ -119,6 +119,9 @@ struct dgeom {
     float max_extent;          // 2*|halfsize|, octree.inl:300
 };
 
+// Work counters of a walk: the loads it issues (node_steps: child words,
+// leaf records, rope records) and the triangle records it tests (a leaf
+// dropped by a cull costs none); the oracle counts the reference's own work.
 struct prop_counters { uint32_t node_steps, tri_tests; };
 
 NBL_HD bool in_domain(const dgeom& g, f3 p)   // octree.inl:24-30
@@ -288,7 +291,7 @@ NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s
         a = { r.cx, r.cy, r.cz };
         h = { ldexpf(g.hx, -(int)(D + 1)), ldexpf(g.hy, -(int)(D + 1)), ldexpf(g.hz, -(int)(D + 1)) };
         level = D;
-        if (COUNT) cnt->node_steps += D;
+        if (COUNT) ++cnt->node_steps;            // the rope record (no descent loads)
         pc->store(level, child);
         node = (uint32_t)child;
         leaf = (child & LEAF_BIT) != 0;
@@ -311,7 +314,6 @@ NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s
             ++level;
             cell_step((unsigned)(location >> (3 * (D - level))) & 7u, a, h);
         }
-        if (COUNT) cnt->node_steps += shared;
         if (same) {
             child = pc->leaf;
             leaf = true;
@@ -351,6 +353,7 @@ NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s
         if (M > 0 && level == M) { child = pc->leaf; leaf = true; }
         else node = pc->node_at(g, level);
     }
+    const uint32_t loads_from = level;
     while (!leaf) {
         const unsigned oct = position_octant(pos, a);
         location = (location << 3) | oct;
@@ -361,7 +364,7 @@ NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s
         leaf = (child & LEAF_BIT) != 0;
         cell_step(oct, a, h);
     }
-    if (COUNT) cnt->node_steps += level - position_from;
+    if (COUNT) cnt->node_steps += level - loads_from;      // child words loaded
     pc->loc = location;
     pc->depth = level;
     pc->leaf = child;
@@ -487,15 +490,12 @@ __device__ __forceinline__ float propagate(const dgeom& g, f3 start, f3 dir, flo
         int axis;          // -1, -2, -4 for x, y, z; 0 once a triangle is kept
         float best = leaf_exit(c, s, pos, dir, axis);
         uint32_t best_id = NO_TRI;
-        if (COUNT) cnt->node_steps += count + 1;
         if (count > 0) {
             const leaf_info li = load_leaf(g, (uint32_t)child);
+            if (COUNT) ++cnt->node_steps;            // the leaf record
             first = li.first;
             bool slab_culled;
-            if (leaf_dropped(li, s, pos, dir, best, to_go, ignore_mat, slab_culled)) {
-                if (COUNT && slab_culled) cnt->tri_tests += count;
-                count = 0;
-            }
+            if (leaf_dropped(li, s, pos, dir, best, to_go, ignore_mat, slab_culled)) count = 0;
         }
         for (uint32_t k = first; k < first + count; ++k) {
             uint32_t id;

@@ -63,15 +63,12 @@ __device__ __forceinline__ bool walk_cell(bool run, const dgeom& g, walk_state& 
         leaf_id = (uint32_t)child;
         count = (uint32_t)((child >> 32) & 0x7FFFFFFFu);
         best = leaf_exit(c, s, w.pos, dir, axis);
-        if (COUNT) cnt->node_steps += count + 1;
         if (count > 0) {
             const leaf_info li = load_leaf(g, (uint32_t)child);
+            if (COUNT) ++cnt->node_steps;        // the leaf record
             first = li.first;
             bool slab_culled;
-            if (leaf_dropped(li, s, w.pos, dir, best, w.to_go, ignore_mat, slab_culled)) {
-                if (COUNT && slab_culled) cnt->tri_tests += count;
-                count = 0;
-            }
+            if (leaf_dropped(li, s, w.pos, dir, best, w.to_go, ignore_mat, slab_culled)) count = 0;
         }
     }
     uint32_t total;

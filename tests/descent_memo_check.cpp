@@ -153,8 +153,9 @@ int main(int argc, char** argv)
         f3 c_a{}, s_a{}, c_b{}, s_b{};
         const uint64_t w_ref = descend<true>(g, pref, l_ref, c_a, s_a, &c_ref, (no_path_cache*)nullptr);
         const uint64_t w_memo = descend<true>(g, pos, l_memo, c_b, s_b, &c_memo, &pc, rope);
+        // the memo's descent loads a subset of the reference-order descent's nodes
         if (w_ref != w_memo || l_ref != l_memo || memcmp(&c_a, &c_b, sizeof c_a) || memcmp(&s_a, &s_b, sizeof s_a) ||
-            c_ref.node_steps != c_memo.node_steps) {
+            c_memo.node_steps > c_ref.node_steps) {
             printf("MISMATCH call %ld kind %d rope %u code %llx: leaf %llx / %llx, code %llx / %llx, steps %u / %u\n",
                 it, kind, rope, (unsigned long long)loc, (unsigned long long)w_ref, (unsigned long long)w_memo,
                 (unsigned long long)l_ref, (unsigned long long)l_memo, c_ref.node_steps, c_memo.node_steps);
@@ -184,7 +185,7 @@ int main(int argc, char** argv)
         }
         if (rng() % 64 == 0) last_loc = 1;
     }
-    printf("ok calls %ld new %ld rope %ld code %ld on_plane %ld node_steps %u max_depth %ld tree_depth %u\n",
-        calls, n_new, n_rope, n_code, n_plane, c_memo.node_steps, max_depth, o.max_depth);
+    printf("ok calls %ld new %ld rope %ld code %ld on_plane %ld node_steps %u ref_node_steps %u max_depth %ld tree_depth %u\n",
+        calls, n_new, n_rope, n_code, n_plane, c_memo.node_steps, c_ref.node_steps, max_depth, o.max_depth);
     return 0;
 }

@@ -39,9 +39,13 @@ def test_bench_line_has_every_field():
         assert k in rl, k
     assert rl["bound"] == "hbm" and rl["unit"] == "GB/s" and rl["peak"] == 8000.0
     assert rl["frac"] == pytest.approx(rl["achieved"] / rl["peak"])
+    # the kernel's own work per second cannot exceed the peak it is held against
+    assert 0 < rl["frac"] <= 1
     # a profile covers the full-size workload only: no traffic figure here
     assert rl["traffic"] is None and rl["traffic_GBps"] is None
     cb = d["cpu_baseline"]
     for k in ("value", "unit", "cores", "kind", "sample"):
         assert k in cb, k
     assert cb["value"] > 0 and cb["kind"] == "port" and cb["cores"] >= 1
+    # every thread the process may run on (cpu_mt_main.cpp:197), with the host stated
+    assert cb["cores"] == cb["host_affinity_threads"] and cb["host_nproc"] >= cb["cores"] and cb["cpu_model"]

@@ -2,7 +2,9 @@
 per-lane path memo and the leaves' ropes) against the reference-order
 descent (descend without a memo: the location code's digits from the root,
 then the position, octree.inl:47-84), on the CPU, call for call: leaf word,
-location code, centre, half size and node-step count must be identical.
+location code, centre and half size must be identical, and the memo's
+descent may load no node the reference-order descent does not (its node-step
+counter counts the child words and rope records it actually loads).
 The reference-order descent is the one the scalar propagate kernel uses,
 which the propagate parity tests hold bit for bit to the oracle.  Scenes:
 the rough surface (9 levels) and a cluster of tiny triangles in a large box
@@ -81,6 +83,7 @@ def test_memo_descent_matches_reference_order(checker, scene, depth):
         f = r.stdout.split()
         stats = dict(zip(f[1::2], (int(x) for x in f[2::2])))
         assert stats["tree_depth"] == depth and stats["max_depth"] == depth
+        assert stats["node_steps"] < stats["ref_node_steps"]
         assert min(stats["new"], stats["rope"], stats["code"]) > 50_000
         assert stats["on_plane"] > 30_000
 
