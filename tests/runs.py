@@ -30,14 +30,73 @@ def oracle_run(og, om, primaries, seed=0x78C7E39B, threads=8, rng_kind=0, min_en
     return {f: getattr(st, f) for f, _ in ol.RunStats._fields_}, hist
 
 
-def yield_z(k1, n1, k2, n2):
-    """z-score of the difference of two yields (detections per primary).
-    Detections per primary are counts with variance estimated by the Poisson
-    bound k/n^2 (cascades give over-dispersion; the per-primary variance is
-    estimated from k as well, conservative for yields below ~1)."""
+def yield_z(k1, n1, k2, n2, dispersion=1.0):
+    """z-score of the difference of two yields (detections per primary) from
+    counts alone, with the per-primary variance taken as dispersion x the
+    yield (1 = Poisson).  A primary's cascade can emit several secondaries,
+    so the true variance is larger (about 1.6 x the SE yield at a line
+    edge): pass the measured ratio, or use per_primary() and z_from() where
+    detection records with primary tags are at hand."""
     y1, y2 = k1 / n1, k2 / n2
-    var = k1 / n1 ** 2 + k2 / n2 ** 2
+    var = dispersion * (k1 / n1 ** 2 + k2 / n2 ** 2)
     return (y1 - y2) / np.sqrt(max(var, 1e-300))
+
+
+def per_primary(energies, tags, n):
+    """SE (E < 50 eV) and BSE detections per primary from detection records
+    (tags = primary index 0..n-1): {"se": (mean, variance), "bse": ...}."""
+    e = np.asarray(energies, np.float32)
+    t = np.asarray(tags, np.int64)
+    se = np.bincount(t[e < 50], minlength=n)[:n]
+    bse = np.bincount(t[e >= 50], minlength=n)[:n]
+    return {"se": (se.mean(), se.var(ddof=1)), "bse": (bse.mean(), bse.var(ddof=1)), "n": n}
+
+
+def z_from(a, b, key):
+    """z-score of the difference of two per_primary() yields, each with its
+    own measured per-primary variance."""
+    (m1, v1), (m2, v2) = a[key], b[key]
+    return (m1 - m2) / np.sqrt(max(v1 / a["n"] + v2 / b["n"], 1e-300))
+
+
+def oracle_records(og, om, primaries, seed=0x78C7E39B, threads=16, rng_kind=0, per_primary_cap=8):
+    """cpu_mt-style oracle run that keeps its detection records: (stats,
+    histogram, detected energies, detected tags), tags = primary index."""
+    n = len(primaries)
+    tags = np.arange(n, dtype=np.uint32)
+    cap = per_primary_cap * n
+    det = np.zeros(cap, L.PARTICLE_DTYPE); dtags = np.zeros(cap, np.uint32)
+    hist = np.zeros(L.HIST_BINS, np.uint64)
+    st = ol.RunStats()
+    rc = ol.orc.orc_cpu_mt_run(og.h, om.h, ol.p(np.ascontiguousarray(primaries)), ol.p(tags), n, 0.0, seed, threads,
+                              rng_kind, 0.0, ol.p(det), ol.p(dtags), cap, ol.p(hist), C.byref(st))
+    assert rc == 0 and st.detected <= cap
+    k = st.detected
+    return {f: getattr(st, f) for f, _ in ol.RunStats._fields_}, hist, det["kin_energy"][:k], dtags[:k]
+
+
+def dispersion(energies, tags, n):
+    """Variance-to-mean ratio of detections per primary (>= 1): one primary's
+    detections are not independent draws, so counting statistics over
+    detections understate the variance by this factor."""
+    c = np.bincount(np.asarray(tags, np.int64), minlength=n)[:n]
+    return max(1.0, float(c.var(ddof=1) / max(c.mean(), 1e-300)))
+
+
+def spectra_agree(h1, h2, label, disp=1.0, p_min=0.0027):
+    """Chi-square (merged bins) and KS on two detected-energy histograms, the
+    statistics deflated by the detections' dispersion (histogram counts of
+    cascades are over-dispersed by about that factor); asserts both p-values
+    exceed p_min (3 sigma, two-sided) and returns them."""
+    from scipy import stats
+    _, chi, dof = chi2_hist(h1, h2)
+    p_chi = stats.chi2.sf(chi / disp, dof)
+    d, _ = ks_binned(h1, h2)
+    n1, n2 = np.sum(h1), np.sum(h2)
+    p_ks = stats.kstwobign.sf(np.sqrt(n1 * n2 / (n1 + n2) / disp) * d)
+    print(f"  {label}: chi2 {chi:.1f}/{dof} (dispersion {disp:.2f}) p {p_chi:.3g}, KS {d:.4f} p {p_ks:.3g}")
+    assert p_chi > p_min and p_ks > p_min, (label, p_chi, p_ks)
+    return p_chi, p_ks
 
 
 def chi2_hist(h1, h2, min_count=20):

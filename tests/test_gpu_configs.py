@@ -1,0 +1,156 @@
+"""Full-run parity of the benchmark configurations against the oracle's CPU
+driver (cpu_driver.inl:73-160 with simple_cpu_driver's run-to-end loop):
+SE and BSE yields within 3 sigma with each side's own per-primary variance
+(a cascade can emit several secondaries, so counts are over-dispersed), and
+chi-square and KS on the detected-energy spectrum (gpu_main.cu:199-206
+writes the detected electrons; their 1-eV histogram is compared), deflated
+by the detections' measured dispersion.
+
+  rough surface   522,242-triangle Gaussian height field, Si, 300 eV
+  stripes8        the same surface in 8 material stripes, 500 eV, table path
+  stripes8        learned BF16 sampler against the table path on the same primaries
+  line scan       the lines geometry, pixels across a line edge (kernel's own Gaussian beam)
+  batched scan    3 materials, 4096 Poisson pixels, 200..2000 eV: a sample of pixels
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import runs
+import nebula_amd as na
+from nebula_amd import _lib as L
+from nebula_amd import scenes, workloads
+from nebula_amd.driver import Driver, raster_scan
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_records(g, m, prim, seed, sampler=None):
+    """Explicit primaries through the persistent kernel with detection
+    records (tags = primary index) and the 1-eV histogram."""
+    n = len(prim)
+    with Driver(g, m, capacity=1024, seed=seed) as d:
+        if sampler is not None:
+            d.set_sampler(sampler)
+        t = d.run_persistent_list(prim, detected=8 * n, hist=True)
+    assert t["primaries"] == n and t["stopped"] == 0 and t["detected_count"] <= 8 * n
+    e, tags = t["detected"][0]["kin_energy"], t["detected"][1]
+    return t, runs.per_primary(e, tags, n), runs.dispersion(e, tags, n)
+
+
+def _compare(label, a, b, ha, hb, disp):
+    for key in ("se", "bse"):
+        z = runs.z_from(a, b, key)
+        print(f"  {label} {key}: {a[key][0]:.5f} vs {b[key][0]:.5f} (var/primary {a[key][1]:.3f}, "
+              f"{b[key][1]:.3f}) z {z:+.2f}")
+        assert abs(z) < 3, (label, key, z)
+    runs.spectra_agree(ha, hb, f"{label} spectrum", disp=disp)
+
+
+@pytest.mark.parametrize("name,n_gpu,n_cpu", [("rough", 1_000_000, 400_000), ("stripes8", 1_000_000, 300_000)])
+def test_full_size_geometry_transport_matches_oracle(name, n_gpu, n_cpu):
+    """The rough surface (300 eV, Si) and its 8-stripe form (500 eV, 8
+    materials, ICDF tables): primaries drawn from the benchmark's own beam."""
+    print(f"[gpu] {name} transport vs oracle", flush=True)
+    tris, specs, beam = workloads.BY_NAME[name]()
+    g = na.Geometry(tris); m = na.Materials(specs)
+    prim_g, _ = workloads.sample_primaries(beam, n_gpu, seed=71)
+    t, a, _ = _gpu_records(g, m, prim_g, seed=5)
+    og = ol.OrcGeometry(tris); om = ol.OrcMaterials([p.desc for p in m.prepared])
+    prim_c, _ = workloads.sample_primaries(beam, n_cpu, seed=72)
+    st, hc, e, tags = runs.oracle_records(og, om, prim_c, threads=16)
+    b = runs.per_primary(e, tags, n_cpu)
+    _compare(name, a, b, t["hist"], hc, runs.dispersion(e, tags, n_cpu))
+
+
+def test_stripes8_learned_sampler_vs_tables():
+    """The learned BF16 sampler (trained as the benchmark trains it, 1e8
+    samples per network) against the ICDF tables on the same 1e6 primaries
+    of the 8-stripe surface: yields within 3 sigma with per-primary
+    variances, spectra by chi-square and KS; the KS distance per energy bin
+    of each network is stated."""
+    print("[gpu] stripes8 learned vs table", flush=True)
+    tris, specs, beam = workloads.stripes8()
+    g = na.Geometry(tris); m = na.Materials(specs)
+    e_beam = float(beam["pixels"]["energy"].max())
+    smp = na.Sampler(m, samples=100_000_000, e_min=0.0, e_max=1.2 * e_beam, seed=5)
+    edges = np.geomspace(5.0, 0.95 * e_beam, 9)
+    for kind, label in ((L.SAMPLER_ELASTIC, "elastic"), (L.SAMPLER_INELASTIC, "inelastic")):
+        ks = np.array([smp.ks_per_bin(kind, mat, edges, n=20000, seed=mat) for mat in range(len(specs))])
+        print(f"  {label} KS per bin: max {np.nanmax(ks):.4f}, median {np.nanmedian(ks):.4f}")
+    prim, _ = workloads.sample_primaries(beam, 1_000_000, seed=73)
+    t_tab, a, disp = _gpu_records(g, m, prim, seed=6)
+    t_mlp, b, _ = _gpu_records(g, m, prim, seed=7, sampler=smp)
+    _compare("learned vs table", b, a, t_mlp["hist"], t_tab["hist"], disp)
+
+
+def test_line_scan_pixel_spectra_match_oracle():
+    """Pixels across one PMMA line of the lines geometry (trench, edge - 1,
+    edge, top) with the kernel's own Gaussian beam (sigma 1 nm): per-pixel
+    yields within 3 sigma, the oracle's measured per-primary dispersion
+    scaling the count variance; pooled spectra by chi-square and KS."""
+    print("[gpu] line scan pixel spectra", flush=True)
+    tris, specs, _ = workloads.lines()
+    g = na.Geometry(tris); m = na.Materials(specs)
+    og = ol.OrcGeometry(tris); om = ol.OrcMaterials([p.desc for p in m.prepared])
+    xs = np.array([-32.0, -17.0, -16.0, 0.0], np.float32)
+    per_gpu, per_cpu = 200_000, 50_000
+    z0 = scenes.LINE_HEIGHT + 5.0
+    with Driver(g, m, capacity=4096, seed=8) as d:
+        t = d.run_persistent(raster_scan(xs, np.zeros(1, np.float32), z0, 500.0, per_gpu, 1.0, seed=4),
+                             per_pixel=True, hist=True)
+    assert t["stopped"] == 0
+    rng = np.random.default_rng(40)
+    h_cpu = np.zeros(L.HIST_BINS, np.uint64)
+    disp_all = []
+    for k, x in enumerate(xs):
+        p = runs.pencil_primaries(per_cpu, 500.0, pos=(float(x), 0.0, z0))
+        p["pos"][:, 0] += rng.normal(0, 1.0, per_cpu); p["pos"][:, 1] += rng.normal(0, 1.0, per_cpu)
+        st, h, e, tags = runs.oracle_records(og, om, p, threads=16, seed=100 + k)
+        h_cpu += h
+        pp = runs.per_primary(e, tags, per_cpu)
+        disp_all.append(runs.dispersion(e, tags, per_cpu))
+        for gk, ck in (("pixel_se", "se"), ("pixel_bse", "bse")):
+            mean, var = pp[ck]
+            ratio = max(1.0, var / max(mean, 1e-12))
+            z = runs.yield_z(int(t[gk][k]), per_gpu, mean * per_cpu, per_cpu, dispersion=ratio)
+            print(f"  x={x:+.0f} {ck}: gpu {t[gk][k] / per_gpu:.4f} cpu {mean:.4f} (var/mean {ratio:.2f}) z {z:+.2f}")
+            assert abs(z) < 3, (x, ck, z)
+    runs.spectra_agree(t["hist"], h_cpu, "line scan (4 pixels)", disp=max(disp_all))
+
+
+def test_batched_scan_sample_spectrum_matches_oracle():
+    """The batched uneven three-material scan in one launch; for a sample of
+    80 pixels, the detections tallied to them (record tags = pixel index)
+    against the oracle on the same pixels: yields and spectrum."""
+    print("[gpu] batched scan sample spectrum", flush=True)
+    tris, specs, beam = workloads.scan3()
+    g = na.Geometry(tris); m = na.Materials(specs)
+    px = beam["pixels"]
+    total = int(px["n_primaries"].sum())
+    with Driver(g, m, capacity=4096, seed=9) as d:
+        t = d.run_persistent(beam, per_pixel=True, detected=8 * total)
+    assert t["primaries"] == total and t["stopped"] == 0 and t["detected_count"] <= 8 * total
+    rng = np.random.default_rng(21)
+    sel = np.sort(rng.choice(len(px), 80, replace=False))
+    e_g, tag_g = t["detected"][0]["kin_energy"], t["detected"][1]
+    bins = e_g[np.isin(tag_g, sel)].astype(np.int64)           # 1-eV bins, as the kernel's histogram
+    h_gpu = np.bincount(bins[(bins >= 0) & (bins < L.HIST_BINS)], minlength=L.HIST_BINS)
+    og = ol.OrcGeometry(tris); om = ol.OrcMaterials([p.desc for p in m.prepared])
+    prims = []
+    for i in sel:
+        n = int(px["n_primaries"][i])
+        p = runs.pencil_primaries(n, float(px["energy"][i]), pos=(float(px["x"][i]), float(px["y"][i]), float(px["z"][i])))
+        p["pos"][:, 0] += rng.normal(0, 1.0, n); p["pos"][:, 1] += rng.normal(0, 1.0, n)
+        prims.append(p)
+    prims = np.concatenate(prims)
+    n_sel = int(px["n_primaries"][sel].sum())
+    st, h_cpu, e, tags = runs.oracle_records(og, om, prims, threads=16)
+    pp = runs.per_primary(e, tags, len(prims))
+    for gk, ck in (("pixel_se", "se"), ("pixel_bse", "bse")):
+        k_gpu = int(t[gk][sel].sum())
+        mean, var = pp[ck]
+        z = runs.yield_z(k_gpu, n_sel, mean * len(prims), len(prims), dispersion=max(1.0, var / max(mean, 1e-12)))
+        print(f"  {ck}: gpu {k_gpu / n_sel:.4f} cpu {mean:.4f} z {z:+.2f}")
+        assert abs(z) < 3, (ck, z)
+    runs.spectra_agree(h_gpu, h_cpu, "batched scan sample", disp=runs.dispersion(e, tags, len(prims)))
