@@ -136,14 +136,47 @@ def table3d_fixture():
     print("table_3D fixture written")
 
 
+def trilist_fixture():
+    """trilist::propagate (geometry/trilist.inl:52-85), the brute-force
+    nearest hit, on a random soup with duplicates: random rays, rays with an
+    ignored triangle or material, and rays started on a triangle along its
+    normal (the ignore rule decides them)."""
+    tris = random_soup(400, 31)
+    lo = np.min(np.concatenate([tris["r0"], tris["r0"] + tris["e1"], tris["r0"] + tris["e2"]]), axis=0)
+    hi = np.max(np.concatenate([tris["r0"], tris["r0"] + tris["e1"], tris["r0"] + tris["e2"]]), axis=0)
+    pos, d, dist, it, im = rays(lo, hi, 8000, 32, len(tris), [0, 1, -123])
+    k = len(tris)
+    c = tris["r0"][:k] + (tris["e1"][:k] + tris["e2"][:k]) / np.float32(3)
+    nrm = np.cross(tris["e1"][:k], tris["e2"][:k]).astype(np.float64)
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1), 1e-30)[:, None]
+    pos[:k] = c.astype(np.float32)
+    d[:k] = np.where(np.arange(k)[:, None] % 2 == 0, nrm, -nrm).astype(np.float32)
+    it[:k] = np.arange(k, dtype=np.uint32)
+    n = len(pos)
+    out_d = np.zeros(n, np.float32); out_t = np.zeros(n, np.uint32)
+    ol.ref.ref_trilist_propagate_batch(ol.p(tris), len(tris), ol.p(pos), ol.p(d), ol.p(dist), ol.p(it), ol.p(im), n,
+                                       ol.p(out_d), ol.p(out_t))
+    np.savez_compressed(os.path.join(HERE, "trilist.npz"), triangles=tris, pos=pos, dir=d, dist=dist, ignore_tri=it,
+                        ignore_mat=im, out_dist=out_d, out_tri=out_t)
+    print("trilist fixture written:", (out_t != NO_TRIANGLE).mean(), "hit fraction")
+
+
+FIXTURES = {
+    "geometry": lambda: (geometry_fixture("cube", scenes.cube_slab(), 4000, 1, [0, -123]),
+                         geometry_fixture("soup", random_soup(600, 2), 6000, 2, [0, 1, -123]),
+                         geometry_fixture("lines", scenes.line_trench(spacing=40.0), 6000, 3, [0, 1, -123])),
+    "tables": table_fixture,
+    "tables3d": table3d_fixture,
+    "trilist": trilist_fixture,
+}
+
+
 def main():
+    """python tests/golden/make_golden.py [fixture ...] (default: all)."""
     if ol.ref is None:
         sys.exit("oracle/_ref/libnbl_ref.so is not built (needs /root/reference): make -C oracle")
-    geometry_fixture("cube", scenes.cube_slab(), 4000, 1, [0, -123])
-    geometry_fixture("soup", random_soup(600, 2), 6000, 2, [0, 1, -123])
-    geometry_fixture("lines", scenes.line_trench(spacing=40.0), 6000, 3, [0, 1, -123])
-    table_fixture()
-    table3d_fixture()
+    for name in sys.argv[1:] or list(FIXTURES):
+        FIXTURES[name]()
 
 
 if __name__ == "__main__":

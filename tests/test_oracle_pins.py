@@ -89,6 +89,21 @@ def test_rough_surface_against_reference_code():
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
 
 
+def test_trilist_matches_reference_fixture():
+    """The oracle's brute force equals the reference's own trilist::propagate
+    (geometry/trilist.inl:52-85, fixture from its source) bit for bit:
+    random rays, ignored triangles and materials, rays leaving a triangle."""
+    f = load("trilist.npz")
+    tris, pos, d, dist, it, im = (f[k] for k in ("triangles", "pos", "dir", "dist", "ignore_tri", "ignore_mat"))
+    n = len(pos)
+    od = np.zeros(n, np.float32); ot = np.zeros(n, np.uint32)
+    ol.orc.orc_trilist_propagate_batch(ol.p(tris), len(tris), ol.p(pos), ol.p(d), ol.p(dist), ol.p(it), ol.p(im), n,
+                                       ol.p(od), ol.p(ot))
+    assert np.array_equal(ot, f["out_tri"])
+    assert np.array_equal(od.view(np.uint32), f["out_dist"].view(np.uint32))
+    assert 0.05 < (ot != NO_TRIANGLE).mean() < 0.95
+
+
 def test_propagate_agrees_with_brute_force():
     """octree::propagate vs trilist::propagate (geometry/trilist.inl:52-85):
     same triangle except where two hits lie within EPSILON (tie rule differs)."""
