@@ -137,9 +137,11 @@ def test_penn_iteration_driver_matches_persistent():
         det = drv.flush_detected()
         assert drv.get_running_count() == 0
     e = np.array([p["kin_energy"] for p, _ in det])
+    a = runs.per_primary(e, np.array([tg for _, tg in det]), n)
     with Driver(g, m, capacity=1 << 16, seed=6) as drv:
-        t = drv.run_persistent(raster_scan(np.zeros(1), np.zeros(1), 5.0, 1000.0, 200_000, 0.0, seed=9))
-    for k_it, k_p in (((e < 50).sum(), t["detected_se"]), ((e >= 50).sum(), t["detected_bse"])):
-        z = runs.yield_z(k_it, n, k_p, t["primaries"])
-        print(f"  iteration {k_it / n:.4f} persistent {k_p / t['primaries']:.4f} z {z:+.2f}")
-        assert abs(z) < 4
+        t = drv.run_persistent_list(runs.pencil_primaries(200_000, 1000.0), detected=1_600_000)
+    b = runs.per_primary(t["detected"][0]["kin_energy"], t["detected"][1], 200_000)
+    for key in ("se", "bse"):
+        z = runs.z_from(a, b, key)
+        print(f"  {key}: iteration {a[key][0]:.4f} persistent {b[key][0]:.4f} z {z:+.2f}")
+        assert abs(z) < 3

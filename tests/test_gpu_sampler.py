@@ -104,7 +104,8 @@ def test_ks_per_energy_bin(trained, kind, bound):
 
 def test_persistent_learned_vs_table_yields(trained):
     """Whole cascades with the learned sampler vs the tables (same kernel,
-    same geometry): SE and BSE yields."""
+    same geometry): SE and BSE yields within 3 sigma, the variance from the
+    spread of 400 pixels' yields."""
     print("[gpu] persistent learned vs table", flush=True)
     m, s = trained
     g = na.Geometry(scenes.cube_slab(material=3))
@@ -120,13 +121,13 @@ def test_persistent_learned_vs_table_yields(trained):
         se = np.hypot(a.std(ddof=1), b.std(ddof=1)) / np.sqrt(len(a))
         z = (b.mean() - a.mean()) / se
         print(f"  {key}: table {a.mean():.4f} learned {b.mean():.4f} z {z:+.2f}")
-        assert abs(b.mean() - a.mean()) < 0.05 * max(a.mean(), 0.02)
+        assert abs(z) < 3, (key, z)
 
 
 def test_elastic_only_sampler(trained):
     """kinds="elastic": only the elastic network is trained and used; the
     inelastic draws stay on the tables (the untrained network is refused),
-    and the transport's yields match the table path's within 4 sigma."""
+    and the transport's yields match the table path's within 3 sigma."""
     print("[gpu] elastic-only sampler", flush=True)
     m, _ = trained
     s = na.Sampler(m, samples=30_000_000, e_min=E_LO, e_max=E_HI, seed=5, kinds="elastic")
@@ -148,7 +149,7 @@ def test_elastic_only_sampler(trained):
         a, b = out["table"][key] / 1000.0, out["learned"][key] / 1000.0
         z = (b.mean() - a.mean()) / (np.hypot(a.std(ddof=1), b.std(ddof=1)) / np.sqrt(len(a)))
         print(f"  {key}: table {a.mean():.4f} learned (elastic) {b.mean():.4f} z {z:+.2f}")
-        assert abs(z) < 4
+        assert abs(z) < 3, (key, z)
 
 
 def test_set_sampler_rejects_other_material_set(trained):
