@@ -228,6 +228,9 @@ def main():
     ap.add_argument("--refill-threshold", type=int, default=0)
     ap.add_argument("--inelastic-batch", type=int, default=0)
     ap.add_argument("--inelastic-wait", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="launch with the library defaults (or the options given) instead of the autotuner's choice")
+    ap.add_argument("--autotune-sample", type=int, default=1_000_000)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
                     help="size of the plain copy kernel run first (counter scale, bandwidth yardstick); 0 = skip")
     args = ap.parse_args()
@@ -265,6 +268,22 @@ def main():
                 inelastic_batch=args.inelastic_batch, inelastic_wait=args.inelastic_wait)
     drv = Driver(g, m, capacity=1024, seed=2024)
     beam = lambda s: dict(base_beam, seed=s)
+    tuned = None
+    if not args.no_autotune and not any(opts.values()):
+        # launch shape and cadence from the autotuner (cached per device,
+        # geometry and beam), which replaces the reference's prescan heuristic
+        from nebula_amd import autotune
+        t_tune = time.time()
+        best, results = autotune.tune(drv, base_beam, sample=args.autotune_sample, repeats=2)
+        default_ms = next((r["kernel_ms"] for r in results
+                           if all(r[k] == v for k, v in autotune.LIBRARY_DEFAULT.items())), None)
+        best_ms = min(r["kernel_ms"] for r in results)
+        opts.update(best)
+        tuned = {"sample_primaries": args.autotune_sample, "candidates": len(results), "chosen": best,
+                 "chosen_sample_ms": best_ms, "library_default_sample_ms": default_ms,
+                 "seconds": round(time.time() - t_tune, 1)}
+        print(f"autotune: {best} ({best_ms:.2f} ms per sample of {args.autotune_sample}; library default "
+              f"{default_ms} ms; {len(results)} candidates)", file=sys.stderr, flush=True)
     for w in range(args.warmup):
         drv.run_persistent(beam(1000 + w), **opts)
         print(f"warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
@@ -342,7 +361,7 @@ def main():
                    "name": args.workload, "primaries_per_step": n_step, "triangles": int(len(tris)),
                    "materials": len(specs), "pixels": int(len(base_beam["pixels"])),
                    "sampler": "table", "kernel": "persistent",
-                   "launch": {k: v for k, v in opts.items()}},
+                   "launch": {k: v for k, v in opts.items()}, "autotune": tuned},
         # HBM roofline of the persistent kernel: algorithmic bytes of the work
         # the launch did (counters x bytes per unit) per second of launch; the
         # HBM bytes it actually moved (profiled) beside it; and the bound the

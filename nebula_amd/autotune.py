@@ -67,8 +67,14 @@ def valid(cfg):
     return lds * cfg["blocks_per_cu"] <= 160 * 1024 and cfg["block_size"] * cfg["blocks_per_cu"] <= 2048
 
 
+LIBRARY_DEFAULT = {"block_size": 256, "blocks_per_cu": 4, "stack_depth": 128, "refill_threshold": 4,
+                   "inelastic_batch": 8}
+
+
 def tune(driver, beam, sample=200_000, space=None, repeats=1, use_cache=True, log=None):
-    """Returns (best_config, results) for this driver and beam."""
+    """Returns (best_config, results) for this driver and beam.  Every
+    candidate of the space that fits a compute unit runs the sample `repeats`
+    times (its best device time counts); results list them all."""
     space = space or DEFAULT_SPACE
     key = _signature(driver, beam, sample)
     cache = {}

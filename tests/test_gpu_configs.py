@@ -19,7 +19,7 @@ import oracle_lib as ol
 import runs
 import nebula_amd as na
 from nebula_amd import _lib as L
-from nebula_amd import scenes, workloads
+from nebula_amd import autotune, scenes, workloads
 from nebula_amd.driver import Driver, raster_scan
 
 pytestmark = pytest.mark.gpu
@@ -120,16 +120,20 @@ def test_line_scan_pixel_spectra_match_oracle():
 
 
 def test_batched_scan_sample_spectrum_matches_oracle():
-    """The batched uneven three-material scan in one launch; for a sample of
-    80 pixels, the detections tallied to them (record tags = pixel index)
-    against the oracle on the same pixels: yields and spectrum."""
+    """The batched uneven three-material scan in one launch, its launch shape
+    and cadence chosen by the autotuner; for a sample of 80 pixels, the
+    detections tallied to them (record tags = pixel index) against the
+    oracle on the same pixels: yields and spectrum."""
     print("[gpu] batched scan sample spectrum", flush=True)
     tris, specs, beam = workloads.scan3()
     g = na.Geometry(tris); m = na.Materials(specs)
     px = beam["pixels"]
     total = int(px["n_primaries"].sum())
     with Driver(g, m, capacity=4096, seed=9) as d:
-        t = d.run_persistent(beam, per_pixel=True, detected=8 * total)
+        # launch shape and cadence from the autotuner, as the configuration asks
+        best, results = autotune.tune(d, beam, sample=100_000)
+        print(f"  autotuned launch {best} of {len(results)} candidates")
+        t = d.run_persistent(beam, per_pixel=True, detected=8 * total, **best)
     assert t["primaries"] == total and t["stopped"] == 0 and t["detected_count"] <= 8 * total
     rng = np.random.default_rng(21)
     sel = np.sort(rng.choice(len(px), 80, replace=False))
