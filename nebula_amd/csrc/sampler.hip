@@ -39,7 +39,10 @@ constexpr int P_W4 = P_B3 + MLP_H;               // [64]
 constexpr int P_B4 = P_W4 + MLP_H;               // [1] (+3 pad)
 constexpr int P_EMB = P_B4 + 4;                  // [MLP_MAX_MAT][8]
 constexpr int P_TOTAL = P_EMB + MLP_MAX_MAT * MLP_EMB;
-constexpr int N_PACK = (4 * 1 + 4 * 4 + 4 * 4) * 64;   // bf16x4 A operands of layers 1..3
+// BF16 A operands, in bf16 elements: layer 1 [m 4][lane 64][4], layers 2 and 3
+// [m 4][q 2][lane 64][8], output [q 2][lane 64][8]
+constexpr int PK_W1 = 0, PK_W2 = PK_W1 + 4 * 64 * 4, PK_W3 = PK_W2 + 8 * 64 * 8, PK_W4 = PK_W3 + 8 * 64 * 8;
+constexpr int N_PACK = PK_W4 + 2 * 64 * 8;
 static_assert(P_TOTAL == NBL_SAMPLER_PARAMS, "parameter block layout");
 static_assert(MLP_KIND_ELASTIC == NBL_SAMPLER_ELASTIC && MLP_KIND_INELASTIC == NBL_SAMPLER_INELASTIC, "kinds");
 
@@ -84,10 +87,9 @@ __global__ __launch_bounds__(256) void k_gen(gen_args a, const float* __restrict
     T[b] = (target_of(a.kind, v) - a.y_shift) / a.y_scale;
     M[b] = mat;
     const size_t B = a.B;
-    X[0 * B + b] = (le - a.e_shift) * a.e_scale;
-    X[1 * B + b] = 2.0f * u - 1.0f;
-    X[2 * B + b] = logf(u) * 0.125f;
-    X[3 * B + b] = logf(fmaxf(1.0f - u, 5.9604644775390625e-8f)) * 0.125f;
+    float sc[4];
+    mlp_scalars(a.e_shift, a.e_scale, le, u, sc);
+    for (int i = 0; i < 4; ++i) X[i * B + b] = sc[i];
     const float* e = prm + P_EMB + mat * MLP_EMB;
     for (int i = 0; i < MLP_EMB; ++i) X[(4 + i) * B + b] = e[i];
     for (int i = 12; i < MLP_IN; ++i) X[i * B + b] = 0.0f;
@@ -258,22 +260,26 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ prm, float* __
     grad[i] = 0;
 }
 
-// BF16 A operands: block (m, q) lane l element e = W[16m + l%16][16q + 4(l/16) + e]
-__global__ void k_pack(const float* __restrict__ prm, bf16x4* __restrict__ out)
+// BF16 A operands in the order of mlp_device.h (one bf16 per thread).
+__global__ void k_pack(const float* __restrict__ prm, uint16_t* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N_PACK) return;
-    int layer, blk = i / 64;
-    const int lane = i % 64;
-    const float* W;
-    int K;
-    if (blk < 4) { layer = 1; W = prm + P_W1; K = MLP_IN; }
-    else if (blk < 20) { layer = 2; blk -= 4; W = prm + P_W2; K = MLP_H; }
-    else { layer = 3; blk -= 20; W = prm + P_W3; K = MLP_H; }
-    (void)layer;
-    const int Q = K / 16, m = blk / Q, q = blk % Q;
-    const int row = 16 * m + lane % 16, col = 16 * q + 4 * (lane / 16);
-    out[i] = pack4(W[row * K + col], W[row * K + col + 1], W[row * K + col + 2], W[row * K + col + 3]);
+    float v;
+    if (i < PK_W2) {                                    // layer 1: [m][lane][e]
+        const int m = i / 256, lane = (i / 4) % 64, e = i % 4;
+        v = prm[P_W1 + (16 * m + lane % 16) * MLP_IN + 4 * (lane / 16) + e];
+    } else if (i < PK_W4) {                             // layers 2, 3: [m][q][lane][e]
+        const bool l2 = i < PK_W3;
+        const int r = i - (l2 ? PK_W2 : PK_W3);
+        const int m = r / 1024, q = (r / 512) % 2, lane = (r / 8) % 64, e = r % 8;
+        v = prm[(l2 ? P_W2 : P_W3) + (16 * m + lane % 16) * MLP_H + mlp_pi(q, lane / 16, e)];
+    } else {                                            // output: [q][lane][e], row 0 only
+        const int r = i - PK_W4;
+        const int q = r / 512, lane = (r / 8) % 64, e = r % 8;
+        v = lane % 16 == 0 ? prm[P_W4 + mlp_pi(q, lane / 16, e)] : 0.0f;
+    }
+    out[i] = to_bf16(v);
 }
 
 // Batch evaluation through the transport's path (mlp_eval_needed): each
@@ -295,15 +301,28 @@ __global__ __launch_bounds__(256) void k_eval(dmlp net, const int* __restrict__ 
     }
 }
 
-// Known-answer test of the matrix-core layout: one 16x16x16 product per lane layout.
-__global__ void k_selftest(const float* __restrict__ A, const float* __restrict__ Bm, float* __restrict__ D)
+// Known-answer tests of the matrix-core layouts: a 16x16x16 product (D1 =
+// A[16][16] B[16][16]) and a 16x16x32 product (D2 = A[16][32] B[32][16]),
+// operands loaded as mlp_device.h assumes.
+__global__ void k_selftest(const float* __restrict__ A, const float* __restrict__ Bm, float* __restrict__ D1,
+    float* __restrict__ D2)
 {
     const int l = threadIdx.x, i = l % 16, g = l / 16;
     const bf16x4 a = pack4(A[i * 16 + 4 * g], A[i * 16 + 4 * g + 1], A[i * 16 + 4 * g + 2], A[i * 16 + 4 * g + 3]);
     const bf16x4 b = pack4(Bm[(4 * g) * 16 + i], Bm[(4 * g + 1) * 16 + i], Bm[(4 * g + 2) * 16 + i], Bm[(4 * g + 3) * 16 + i]);
     f32x4 acc = { 0, 0, 0, 0 };
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    for (int r = 0; r < 4; ++r) D[(4 * g + r) * 16 + i] = acc[r];
+    for (int r = 0; r < 4; ++r) D1[(4 * g + r) * 16 + i] = acc[r];
+    // 16x16x32: A row i = l % 16, k = 8g + e; B col j = l % 16, k = 8g + e
+    const float* A2 = A + 256;                 // [16][32]
+    const float* B2 = Bm + 256;                // [32][16]
+    bf16x4 alo = pack4(A2[i * 32 + 8 * g], A2[i * 32 + 8 * g + 1], A2[i * 32 + 8 * g + 2], A2[i * 32 + 8 * g + 3]);
+    bf16x4 ahi = pack4(A2[i * 32 + 8 * g + 4], A2[i * 32 + 8 * g + 5], A2[i * 32 + 8 * g + 6], A2[i * 32 + 8 * g + 7]);
+    bf16x4 blo = pack4(B2[(8 * g) * 16 + i], B2[(8 * g + 1) * 16 + i], B2[(8 * g + 2) * 16 + i], B2[(8 * g + 3) * 16 + i]);
+    bf16x4 bhi = pack4(B2[(8 * g + 4) * 16 + i], B2[(8 * g + 5) * 16 + i], B2[(8 * g + 6) * 16 + i], B2[(8 * g + 7) * 16 + i]);
+    f32x4 acc2 = { 0, 0, 0, 0 };
+    acc2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(join8(alo, ahi), join8(blo, bhi), acc2, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) D2[(4 * g + r) * 16 + i] = acc2[r];
 }
 
 }  // namespace
@@ -312,7 +331,7 @@ struct nbl_sampler_s {
     nbl_materials mats = nullptr;
     int n_mat = 0;
     dbuf<float> prm[2];            // FP32 parameters per network
-    dbuf<bf16x4> packed[2];        // BF16 A operands per network
+    dbuf<uint16_t> packed[2];      // BF16 A operands per network
     dmlp view[2]{};
     uint32_t kinds = 3;            // bit k: network k trained and used
     nbl_sampler_report report{};
@@ -330,8 +349,12 @@ void make_view(nbl_sampler_s& s, int k, float e_shift, float e_scale, float y_sh
 {
     dmlp& v = s.view[k];
     const float* p = s.prm[k].p;
-    v.w1 = s.packed[k].p; v.w2 = v.w1 + 4 * 64; v.w3 = v.w2 + 16 * 64;
-    v.b1 = p + P_B1; v.b2 = p + P_B2; v.b3 = p + P_B3; v.w4 = p + P_W4; v.emb = p + P_EMB;
+    const uint16_t* pk = s.packed[k].p;
+    v.w1 = reinterpret_cast<const bf16x4*>(pk + PK_W1);
+    v.w2 = reinterpret_cast<const bf16x8*>(pk + PK_W2);
+    v.w3 = reinterpret_cast<const bf16x8*>(pk + PK_W3);
+    v.w4 = reinterpret_cast<const bf16x8*>(pk + PK_W4);
+    v.b1 = p + P_B1; v.b2 = p + P_B2; v.b3 = p + P_B3; v.emb = p + P_EMB;
     v.b4 = b4; v.e_shift = e_shift; v.e_scale = e_scale; v.y_shift = y_shift; v.y_scale = y_scale;
     v.kind = k;
 }
@@ -535,22 +558,26 @@ int nbl_sampler_selftest(double* max_abs_err)
         require(max_abs_err != nullptr, "nbl_sampler_selftest: NULL argument");
         ensure_device();
         // small integers and halves: exact in BF16 and in the FP32 sums
-        std::vector<float> A(256), Bm(256), D(256);
-        for (int i = 0; i < 256; ++i) {
+        // A: [16][16] then [16][32]; B: [16][16] then [32][16]
+        std::vector<float> A(256 + 512), Bm(256 + 512), D1(256), D2(256);
+        for (int i = 0; i < 768; ++i) {
             A[i] = (float)((i * 7) % 9 - 4) * 0.5f;
             Bm[i] = (float)((i * 5) % 11 - 5) * 0.25f;
         }
-        dbuf<float> dA(256), dB(256), dD(256);
-        dA.upload(A.data(), 256); dB.upload(Bm.data(), 256);
-        hipLaunchKernelGGL(k_selftest, dim3(1), dim3(64), 0, 0, dA.p, dB.p, dD.p);
+        dbuf<float> dA(768), dB(768), dD1(256), dD2(256);
+        dA.upload(A.data(), 768); dB.upload(Bm.data(), 768);
+        hipLaunchKernelGGL(k_selftest, dim3(1), dim3(64), 0, 0, dA.p, dB.p, dD1.p, dD2.p);
         NBL_HIP(hipGetLastError());
-        dD.download(D.data(), 256);
+        dD1.download(D1.data(), 256);
+        dD2.download(D2.data(), 256);
+        NBL_HIP(hipDeviceSynchronize());
         double err = 0;
         for (int i = 0; i < 16; ++i)
             for (int j = 0; j < 16; ++j) {
-                double ref = 0;
-                for (int k = 0; k < 16; ++k) ref += (double)A[i * 16 + k] * Bm[k * 16 + j];
-                err = std::max(err, std::fabs(ref - D[i * 16 + j]));
+                double r1 = 0, r2 = 0;
+                for (int k = 0; k < 16; ++k) r1 += (double)A[i * 16 + k] * Bm[k * 16 + j];
+                for (int k = 0; k < 32; ++k) r2 += (double)A[256 + i * 32 + k] * Bm[256 + k * 16 + j];
+                err = std::max(err, std::max(std::fabs(r1 - D1[i * 16 + j]), std::fabs(r2 - D2[i * 16 + j])));
             }
         *max_abs_err = err;
     });

@@ -46,8 +46,8 @@ def _bf16(x):
 
 
 def _numpy_forward(p, c, kind, mat, energy, u):
-    """The network of mlp_device.h in NumPy: BF16 weights and activations,
-    FP32 sums (parameter layout of NBL_SAMPLER_PARAMS in include/nbl_hip.h)."""
+    """The network of mlp_device.h in NumPy: BF16 weights (the output layer's
+    too) and activations, FP32 sums (parameter layout of NBL_SAMPLER_PARAMS in include/nbl_hip.h)."""
     n = len(energy)
     f = np.zeros((n, 16), np.float32)
     f[:, 0] = (np.log(energy).astype(np.float32) - c[0]) * c[1]
@@ -62,7 +62,7 @@ def _numpy_forward(p, c, kind, mat, energy, u):
     h = np.maximum(_bf16(f) @ _bf16(w1).T + b1, 0)
     h = np.maximum(_bf16(h) @ _bf16(w2).T + b2, 0)
     h = np.maximum(_bf16(h) @ _bf16(w3).T + b3, 0)
-    v = (_bf16(h) @ w4 + b4) * c[3] + c[2]
+    v = (_bf16(h) @ _bf16(w4) + b4) * c[3] + c[2]
     return 1.000001 - np.exp(v) if kind == L.SAMPLER_ELASTIC else v
 
 
