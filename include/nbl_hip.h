@@ -393,6 +393,23 @@ typedef struct {
     uint64_t count;          /* out                                                    */
 } nbl_detection_buffer;
 
+/* Energy-deposition events as records, as cpu_energydep.cpp:184-192 writes
+ * them before the pixel lookup: the event position (the particle after the
+ * event), its energy before and the energy it lost, in event order of
+ * completion; tags are the primaries' tags.  count is every deposit event;
+ * only the first min(count, capacity) are stored (capacity 0: count only). */
+typedef struct {
+    nbl_vec3 pos;            /* event position, nm                                     */
+    float energy;            /* kinetic energy before the event, eV                    */
+    float deposit;           /* energy before - energy after, eV (> 0)                 */
+} nbl_deposit_record;        /* 20 B */
+typedef struct {
+    nbl_deposit_record* records; /* [capacity] out                                     */
+    uint32_t* tags;              /* [capacity] out                                     */
+    uint64_t capacity;
+    uint64_t count;              /* out                                                 */
+} nbl_deposit_buffer;
+
 typedef struct {
     uint64_t primaries;      /* primaries whose cascades ended                    */
     uint64_t detected_se;    /* detected with E < 50 eV                            */
@@ -409,6 +426,7 @@ typedef struct {
     uint32_t* hist;          /* optional [NBL_HIST_BINS] detected-energy histogram, 1-eV bins */
     nbl_deposit_grid* deposit; /* optional energy-deposition tally (may be NULL)      */
     nbl_detection_buffer* detected; /* optional detected-electron records (may be NULL) */
+    nbl_deposit_buffer* deposits;   /* optional energy-deposition records (may be NULL)  */
 } nbl_tallies;
 
 typedef struct {

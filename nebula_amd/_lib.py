@@ -146,11 +146,18 @@ class Tallies(C.Structure):
                 ("boundary_events", C.c_uint64), ("secondaries", C.c_uint64),
                 ("pending_retries", C.c_uint64), ("stopped", C.c_uint64), ("kernel_ms", C.c_double),
                 ("pixel_se", C.c_void_p), ("pixel_bse", C.c_void_p), ("hist", C.c_void_p),
-                ("deposit", C.c_void_p), ("detected", C.c_void_p)]
+                ("deposit", C.c_void_p), ("detected", C.c_void_p), ("deposits", C.c_void_p)]
 
 
 class DetectionBuffer(C.Structure):
     _fields_ = [("particles", C.c_void_p), ("tags", C.c_void_p), ("capacity", C.c_uint64), ("count", C.c_uint64)]
+
+
+class DepositBuffer(C.Structure):
+    _fields_ = [("records", C.c_void_p), ("tags", C.c_void_p), ("capacity", C.c_uint64), ("count", C.c_uint64)]
+
+
+DEPOSIT_RECORD_DTYPE = np.dtype([("pos", "<f4", 3), ("energy", "<f4"), ("deposit", "<f4")])
 
 
 class DepositGrid(C.Structure):

@@ -49,6 +49,7 @@ struct nbl_driver_s {
     uint64_t runs = 0;                  // persistent launches so far (stream offset)
     unsigned tpb = 256;
     hipStream_t stream = nullptr, copy_stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;   // persistent launch timing
 
     nbl::dbuf<uint8_t> status;
     nbl::dbuf<uint32_t> pidx;

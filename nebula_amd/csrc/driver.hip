@@ -301,6 +301,8 @@ nbl_driver_s::~nbl_driver_s()
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
     host_free(hin_data); host_free(hin_parts); host_free(hin_tags);
     host_free(hout_status); host_free(hout_parts); host_free(hout_tags);
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_stop) (void)hipEventDestroy(ev_stop);
     if (stream) (void)hipStreamDestroy(stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
 }
@@ -329,6 +331,8 @@ int nbl_driver_create(uint32_t capacity, nbl_geometry g, nbl_materials m, float 
         }
         NBL_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
         NBL_HIP(hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking));
+        NBL_HIP(hipEventCreate(&d->ev_start));
+        NBL_HIP(hipEventCreate(&d->ev_stop));
         d->status.alloc(capacity); d->pidx.alloc(capacity); d->parts.alloc(capacity);
         d->tags.alloc(capacity); d->mat.alloc(capacity); d->last_tri.alloc(capacity); d->rng.alloc(capacity);
         d->sort.init(capacity);

@@ -51,6 +51,9 @@ struct dpixel {
     uint32_t tally;                     // index of the caller's pixel
 };
 
+// One energy-deposition event (nbl_deposit_record).
+struct drecord { f3 pos; float energy, deposit; };
+
 struct pers_params {
     dgeom g;
     const dmat* mats;
@@ -83,9 +86,14 @@ struct pers_params {
     uint32_t* det_tags;
     unsigned long long* det_count;      // all detections (may exceed det_cap)
     uint64_t det_cap;
-    // energy-deposition tally (dep == nullptr: off)
+    // energy-deposition tally (DEPOSIT): grid (dep == nullptr: off) and records (drec_on == 0: off)
     double* dep;                        // [dnz][dny][dnx] voxels
-    double* dep_total;                  // all deposits
+    double* dep_total;                  // all deposits (nullptr: not summed)
+    uint32_t drec_on;
+    drecord* drec;
+    uint32_t* drec_tags;
+    unsigned long long* drec_count;     // all deposit events (may exceed drec_cap)
+    uint64_t drec_cap;
     uint32_t dnx, dny, dnz;
     f3 dlo, dinv;                       // voxel index = floor((p - dlo) * dinv)
 };
@@ -358,14 +366,26 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
             if (DEPOSIT) {
                 const float de = (el || go) ? e_before - p.e : 0.0f;
                 const bool dep = de != 0.0f;
-                n_dep += __popcll(__ballot(dep));
+                const uint64_t dm = __ballot(dep);
+                n_dep += __popcll(dm);
                 if (dep) {
                     dep_sum += de;
                     const int ix = (int)floorf((p.pos.x - P.dlo.x) * P.dinv.x);
                     const int iy = (int)floorf((p.pos.y - P.dlo.y) * P.dinv.y);
                     const int iz = (int)floorf((p.pos.z - P.dlo.z) * P.dinv.z);
-                    if (ix >= 0 && iy >= 0 && iz >= 0 && ix < (int)P.dnx && iy < (int)P.dny && iz < (int)P.dnz)
+                    if (P.dep && ix >= 0 && iy >= 0 && iz >= 0 && ix < (int)P.dnx && iy < (int)P.dny &&
+                        iz < (int)P.dnz)
                         atomicAdd(&P.dep[((size_t)iz * P.dny + iy) * P.dnx + ix], (double)de);
+                }
+                // records as cpu_energydep.cpp:184-192 writes them: one counter update per wavefront
+                if (P.drec_on && dm) {
+                    unsigned long long base = 0;
+                    if (lane == 0) base = atomicAdd(P.drec_count, (unsigned long long)__popcll(dm));
+                    base = bcast64(base);
+                    if (dep) {
+                        const uint64_t k = base + rank_below(dm);
+                        if (k < P.drec_cap) { P.drec[k] = drecord{ p.pos, e_before, de }; P.drec_tags[k] = tag; }
+                    }
                 }
             }
         }
@@ -408,7 +428,7 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
         }
         if (lane == 0 && v) atomicAdd(&P.counters[k], (unsigned long long)v);
     }
-    if (DEPOSIT) {
+    if (DEPOSIT && P.dep_total) {
         for (int off = 32; off > 0; off >>= 1) dep_sum += __shfl_xor(dep_sum, off);
         if (lane == 0 && dep_sum != 0) atomicAdd(P.dep_total, dep_sum);
     }
@@ -427,36 +447,118 @@ struct primary_source {
     uint64_t n_list = 0;
 };
 
+// Launch shape and cadence of a persistent run (nbl_persistent_opts; 0 = default).
+struct launch_shape {
+    int cus = 0;
+    uint32_t block = 256, per_cu = 4, depth = 128, thresh = 4, ovf = 16384, ibatch = 8, iwait = 8;
+    bool ovf_given = false, allow_drops = false, penn = false;
+    size_t lds = 0;
+    size_t n_waves() const { return (size_t)cus * per_cu * (block / 64); }
+    uint64_t n_threads() const { return (uint64_t)cus * per_cu * block; }
+};
+
+launch_shape resolve_shape(nbl_driver d, const nbl_persistent_opts* opts)
+{
+    launch_shape L;
+    int dev = 0;
+    NBL_HIP(hipGetDevice(&dev));
+    NBL_HIP(hipDeviceGetAttribute(&L.cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (opts && opts->block_size) L.block = opts->block_size;
+    // the learned variant holds 3 wavefronts per SIMD (12 per CU), the table variant 4
+    L.per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : (d->sampler ? 3u : 4u);
+    if (opts && opts->stack_depth) L.depth = opts->stack_depth;
+    if (opts && opts->refill_threshold) L.thresh = opts->refill_threshold;
+    L.ovf_given = opts && opts->overflow_depth;
+    if (L.ovf_given) L.ovf = opts->overflow_depth;
+    if (opts && opts->inelastic_batch) L.ibatch = opts->inelastic_batch;
+    if (opts && opts->inelastic_wait) L.iwait = opts->inelastic_wait;
+    L.allow_drops = opts && opts->allow_drops;
+    require(L.ibatch <= 64, "nbl_driver_run_persistent: inelastic_batch must be 1..64");
+    require(L.iwait <= 1024, "nbl_driver_run_persistent: inelastic_wait must be <= 1024");
+    require(L.ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
+    require(L.block % 64 == 0 && L.block >= 64 && L.block <= 1024,
+        "nbl_driver_run_persistent: block_size must be 64..1024, multiple of 64");
+    require(!d->sampler || L.block <= 768, "nbl_driver_run_persistent: block_size must be <= 768 with a learned sampler");
+    L.penn = materials_view(d->mats, 0).model == INEL_PENN;
+    require(!(L.penn && d->sampler), "nbl_driver_run_persistent: the learned sampler covers Kieft inelastic tables only");
+    require(L.per_cu >= 1 && L.per_cu <= 32, "nbl_driver_run_persistent: blocks_per_cu must be 1..32");
+    require(L.depth >= 64 && L.depth <= 1024, "nbl_driver_run_persistent: stack_depth must be 64..1024");
+    require(L.thresh >= 1 && L.thresh <= 64, "nbl_driver_run_persistent: refill_threshold must be 1..64");
+    L.lds = (size_t)(L.block / 64) * (9 * L.depth + lds_path_cache::kLevels * 64) * sizeof(float);
+    require(L.lds <= 160 * 1024, "nbl_driver_run_persistent: stack_depth x waves exceeds the LDS size");
+    return L;
+}
+
+// The parameters every persistent launch shares (geometry, materials, shape,
+// RNG streams); the caller sets the primaries, tallies and scratch pointers.
+pers_params base_params(nbl_driver d, const launch_shape& L)
+{
+    pers_params P{};
+    P.g = geometry_view(d->geom);
+    P.mats = materials_device(d->mats);
+    P.min_e = d->min_energy; P.max_e = d->max_energy;
+    P.rng_seed = d->seed;
+    P.rng_first = (uint64_t)d->cap + (d->runs++) * L.n_threads();   // streams distinct from the driver's slots
+    P.depth = L.depth; P.ovf = L.ovf; P.refill_threshold = L.thresh;
+    P.inel_batch = L.ibatch; P.inel_wait = L.iwait;
+    if (d->sampler) {
+        const dmlp* v = sampler_views(d->sampler);
+        P.mlp[0] = v[0]; P.mlp[1] = v[1];
+        P.learned = sampler_kinds(d->sampler);
+    }
+    return P;
+}
+
+// Launches the transport on the driver's stream between the driver's two
+// events and returns the launch's device time in ms (waits for it).
+float launch_transport(nbl_driver d, const launch_shape& L, const pers_params& P, bool deposit)
+{
+    hipStream_t s = d->stream;
+    NBL_HIP(hipEventRecord(d->ev_start, s));
+    const dim3 grid((unsigned)(L.cus * L.per_cu)), threads_per_block(L.block);
+    if (d->sampler) {
+        if (deposit) hipLaunchKernelGGL((k_persistent<true, true, false>), grid, threads_per_block, L.lds, s, P);
+        else hipLaunchKernelGGL((k_persistent<true, false, false>), grid, threads_per_block, L.lds, s, P);
+    } else if (L.penn) {
+        if (deposit) hipLaunchKernelGGL((k_persistent<false, true, true>), grid, threads_per_block, L.lds, s, P);
+        else hipLaunchKernelGGL((k_persistent<false, false, true>), grid, threads_per_block, L.lds, s, P);
+    } else {
+        if (deposit) hipLaunchKernelGGL((k_persistent<false, true, false>), grid, threads_per_block, L.lds, s, P);
+        else hipLaunchKernelGGL((k_persistent<false, false, false>), grid, threads_per_block, L.lds, s, P);
+    }
+    NBL_HIP(hipGetLastError());
+    NBL_HIP(hipEventRecord(d->ev_stop, s));
+    NBL_HIP(hipEventSynchronize(d->ev_stop));
+    float ms = 0;
+    NBL_HIP(hipEventElapsedTime(&ms, d->ev_start, d->ev_stop));
+    return ms;
+}
+
+// Counters of a finished launch into *out; a run that stopped electrons is
+// an error unless the caller allowed drops.
+void finish_tallies(const uint64_t* cnt, const unsigned long long* hist, float ms, const launch_shape& L,
+    nbl_tallies* out)
+{
+    if (out->hist) for (int b = 0; b < NBL_HIST_BINS; ++b) out->hist[b] = (uint32_t)hist[b];
+    out->primaries = cnt[C_PRIM]; out->detected_se = cnt[C_SE]; out->detected_bse = cnt[C_BSE];
+    out->events = cnt[C_EVENTS]; out->lookups_1d = cnt[C_L1D]; out->lookups_2d = cnt[C_L2D];
+    out->rounddowns = cnt[C_RD]; out->node_steps = cnt[C_NODES]; out->tri_tests = cnt[C_TRIS];
+    out->boundary_events = cnt[C_BOUND]; out->secondaries = cnt[C_SEC]; out->pending_retries = cnt[C_PEND];
+    out->stopped = cnt[C_KILLED];
+    out->kernel_ms = ms;
+    if (out->stopped && !L.allow_drops)
+        throw std::runtime_error("nbl_driver_run_persistent: " + std::to_string(out->stopped) +
+            " electrons stopped (secondaries dropped for lack of stack room, or the 2^20-step cap); the tallies "
+            "are biased. Raise overflow_depth, or set allow_drops to accept them");
+}
+
 void run_persistent(nbl_driver d, const primary_source& src, const nbl_persistent_opts* opts, nbl_tallies* out)
 {
     const uint64_t total = src.list ? src.n_list : src.offsets.back();
     const uint32_t nkp = (uint32_t)src.kp.size();
     const uint32_t np = src.n_tally;
     require(total > 0, "nbl_driver_run_persistent: no primaries in the batch");
-
-    int dev = 0, cus = 0;
-    NBL_HIP(hipGetDevice(&dev));
-    NBL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    uint32_t block = opts && opts->block_size ? opts->block_size : 256;
-    // the learned variant holds 3 wavefronts per SIMD (12 per CU), the table variant 4
-    uint32_t per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : (d->sampler ? 3u : 4u);
-    uint32_t depth = opts && opts->stack_depth ? opts->stack_depth : 128;
-    uint32_t thresh = opts && opts->refill_threshold ? opts->refill_threshold : 4;
-    uint32_t ovf = opts && opts->overflow_depth ? opts->overflow_depth : 16384;
-    const uint32_t ibatch = opts && opts->inelastic_batch ? opts->inelastic_batch : 8;
-    const uint32_t iwait = opts && opts->inelastic_wait ? opts->inelastic_wait : 8;
-    require(ibatch <= 64, "nbl_driver_run_persistent: inelastic_batch must be 1..64");
-    require(iwait <= 1024, "nbl_driver_run_persistent: inelastic_wait must be <= 1024");
-    require(ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
-    require(block % 64 == 0 && block >= 64 && block <= 1024, "nbl_driver_run_persistent: block_size must be 64..1024, multiple of 64");
-    require(!d->sampler || block <= 768, "nbl_driver_run_persistent: block_size must be <= 768 with a learned sampler");
-    const bool penn = materials_view(d->mats, 0).model == INEL_PENN;
-    require(!(penn && d->sampler), "nbl_driver_run_persistent: the learned sampler covers Kieft inelastic tables only");
-    require(per_cu >= 1 && per_cu <= 32, "nbl_driver_run_persistent: blocks_per_cu must be 1..32");
-    require(depth >= 64 && depth <= 1024, "nbl_driver_run_persistent: stack_depth must be 64..1024");
-    require(thresh >= 1 && thresh <= 64, "nbl_driver_run_persistent: refill_threshold must be 1..64");
-    const size_t lds = (size_t)(block / 64) * (9 * depth + lds_path_cache::kLevels * 64) * sizeof(float);
-    require(lds <= 160 * 1024, "nbl_driver_run_persistent: stack_depth x waves exceeds the LDS size");
+    const launch_shape L = resolve_shape(d, opts);
 
     dbuf<dpixel> dpix(std::max<uint32_t>(nkp, 1));
     dbuf<uint64_t> doff(nkp + 1);
@@ -490,10 +592,7 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
     NBL_HIP(hipMemsetAsync(dse.p, 0, std::max<uint32_t>(np, 1) * sizeof(uint32_t), s));
     NBL_HIP(hipMemsetAsync(dbse.p, 0, std::max<uint32_t>(np, 1) * sizeof(uint32_t), s));
 
-    pers_params P;
-    P.g = geometry_view(d->geom);
-    P.mats = materials_device(d->mats);
-    P.min_e = d->min_energy; P.max_e = d->max_energy;
+    pers_params P = base_params(d, L);
     P.pixels = dpix.p; P.offsets = doff.p; P.n_pixels = nkp; P.total = total;
     P.plist = src.list ? dlist.p : nullptr; P.ptags = src.list ? dltags.p : nullptr;
     P.det_on = db ? 1u : 0u;             // capacity 0: count only
@@ -502,17 +601,11 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
     for (uint32_t q = 0; q <= N_QUEUES; ++q) P.qbeg[q] = total * q / N_QUEUES;   // contiguous eighths
     P.dir = src.dir;
     P.prim_seed = src.seed;
-    P.rng_seed = d->seed;
-    const uint64_t threads = (uint64_t)cus * per_cu * block;
-    P.rng_first = (uint64_t)d->cap + (d->runs++) * threads;   // streams distinct from the driver's slots
     P.next = dnext.p; P.hist = dhist.p; P.counters = dcnt.p;
     P.pixel_se = np ? dse.p : nullptr; P.pixel_bse = np ? dbse.p : nullptr;
-    const size_t n_waves = (size_t)cus * per_cu * (block / 64);
-    const size_t ovf_floats = n_waves * 9 * ovf;
+    const size_t ovf_floats = L.n_waves() * 9 * L.ovf;
     if (d->overflow.n < ovf_floats) d->overflow.alloc(ovf_floats);
     P.overflow = d->overflow.p;
-    P.depth = depth; P.ovf = ovf; P.refill_threshold = thresh;
-    P.inel_batch = ibatch; P.inel_wait = iwait;
     // energy-deposition tally
     nbl_deposit_grid* dg = out->deposit;
     dbuf<double> ddep, dtot;
@@ -534,29 +627,22 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
                    (float)dg->nz / (dg->hi.z - dg->lo.z) };
     }
 
-    hipEvent_t e0, e1;
-    NBL_HIP(hipEventCreate(&e0)); NBL_HIP(hipEventCreate(&e1));
-    NBL_HIP(hipEventRecord(e0, s));
-    const dim3 grid((unsigned)(cus * per_cu)), threads_per_block(block);
-    if (d->sampler) {
-        const dmlp* v = sampler_views(d->sampler);
-        P.mlp[0] = v[0]; P.mlp[1] = v[1];
-        P.learned = sampler_kinds(d->sampler);
-        if (dg) hipLaunchKernelGGL((k_persistent<true, true, false>), grid, threads_per_block, lds, s, P);
-        else hipLaunchKernelGGL((k_persistent<true, false, false>), grid, threads_per_block, lds, s, P);
-    } else if (penn) {
-        if (dg) hipLaunchKernelGGL((k_persistent<false, true, true>), grid, threads_per_block, lds, s, P);
-        else hipLaunchKernelGGL((k_persistent<false, false, true>), grid, threads_per_block, lds, s, P);
-    } else {
-        if (dg) hipLaunchKernelGGL((k_persistent<false, true, false>), grid, threads_per_block, lds, s, P);
-        else hipLaunchKernelGGL((k_persistent<false, false, false>), grid, threads_per_block, lds, s, P);
+    // energy-deposition records
+    nbl_deposit_buffer* drb = out->deposits;
+    dbuf<drecord> ddrec;
+    dbuf<uint32_t> ddrtags;
+    dbuf<unsigned long long> ddrcnt(1);
+    NBL_HIP(hipMemsetAsync(ddrcnt.p, 0, sizeof(unsigned long long), s));
+    P.drec_on = 0; P.drec = nullptr; P.drec_tags = nullptr; P.drec_count = ddrcnt.p; P.drec_cap = 0;
+    if (drb) {
+        static_assert(sizeof(drecord) == sizeof(nbl_deposit_record), "deposit record layout");
+        require(drb->capacity == 0 || (drb->records && drb->tags),
+            "nbl_driver_run_persistent: deposit record buffer needs record and tag arrays");
+        if (drb->capacity) { ddrec.alloc(drb->capacity); ddrtags.alloc(drb->capacity); }
+        P.drec_on = 1; P.drec = ddrec.p; P.drec_tags = ddrtags.p; P.drec_cap = drb->capacity;
     }
-    NBL_HIP(hipGetLastError());
-    NBL_HIP(hipEventRecord(e1, s));
-    NBL_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    NBL_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+
+    const float ms = launch_transport(d, L, P, dg != nullptr || drb != nullptr);
 
     uint64_t cnt[N_COUNTERS];
     dcnt.download(reinterpret_cast<unsigned long long*>(cnt), N_COUNTERS, s);
@@ -579,19 +665,20 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
         ddep.download(dg->energy, (size_t)dg->nx * dg->ny * dg->nz, s);
         dtot.download(&dg->total, 1, s);
     }
+    if (drb) {
+        unsigned long long n_rec = 0;
+        ddrcnt.download(&n_rec, 1, s);
+        NBL_HIP(hipStreamSynchronize(s));
+        drb->count = n_rec;
+        const uint64_t k = std::min<uint64_t>(n_rec, drb->capacity);
+        if (k) {
+            ddrec.download(reinterpret_cast<drecord*>(drb->records), k, s);
+            ddrtags.download(drb->tags, k, s);
+        }
+    }
     NBL_HIP(hipStreamSynchronize(s));
-    if (out->hist) for (int b = 0; b < NBL_HIST_BINS; ++b) out->hist[b] = (uint32_t)hist[b];
-    out->primaries = cnt[C_PRIM]; out->detected_se = cnt[C_SE]; out->detected_bse = cnt[C_BSE];
-    out->events = cnt[C_EVENTS]; out->lookups_1d = cnt[C_L1D]; out->lookups_2d = cnt[C_L2D];
-    out->rounddowns = cnt[C_RD]; out->node_steps = cnt[C_NODES]; out->tri_tests = cnt[C_TRIS];
-    out->boundary_events = cnt[C_BOUND]; out->secondaries = cnt[C_SEC]; out->pending_retries = cnt[C_PEND];
-    out->stopped = cnt[C_KILLED];
     if (dg) dg->count = cnt[C_DEPN];
-    out->kernel_ms = ms;
-    if (out->stopped && !(opts && opts->allow_drops))
-        throw std::runtime_error("nbl_driver_run_persistent: " + std::to_string(out->stopped) +
-            " electrons stopped (secondaries dropped for lack of stack room, or the 2^20-step cap); the tallies "
-            "are biased. Raise overflow_depth, or set allow_drops to accept them");
+    finish_tallies(cnt, hist.data(), ms, L, out);
 }
 
 }  // namespace

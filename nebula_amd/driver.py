@@ -159,13 +159,16 @@ class Driver:
 
     def run_persistent(self, beam, block_size=0, blocks_per_cu=0, stack_depth=0, refill_threshold=0,
                        per_pixel=False, hist=False, deposit=None, detected=0, inelastic_batch=0, inelastic_wait=0,
-                       overflow_depth=0, allow_drops=False):
+                       overflow_depth=0, allow_drops=False, deposit_records=None):
         """One persistent launch over the pixel batch `beam`.  deposit =
         dict(shape=(nx, ny, nz), lo=(x, y, z), hi=(x, y, z)) turns on the
         energy-deposition tally: out["deposit"] is the [nz, ny, nx] grid (eV),
         out["deposit_total"] and out["deposit_count"] cover every deposit.
         detected = capacity of the detected-electron record buffer (0: off):
         out["detected"] = (particles, tags), out["detected_count"].
+        deposit_records = capacity of the energy-deposition record buffer
+        (None: off; 0: count only): out["deposit_records"] = (records of
+        DEPOSIT_RECORD_DTYPE, tags), out["deposit_records_count"].
         inelastic_batch, inelastic_wait, overflow_depth: nbl_persistent_opts (0 = defaults).
         A run that stops any electron (a secondary with no stack room left, or
         the step cap) raises NblError unless allow_drops; then out["stopped"]
@@ -175,11 +178,11 @@ class Driver:
         b = L.PixelBatch(pixels=ptr(px), n_pixels=len(px), dir=L.Vec3(*d), seed=int(beam.get("seed", 0)))
         return self._persistent("nbl_driver_run_persistent", (C.byref(b),), len(px), block_size, blocks_per_cu,
                                 stack_depth, refill_threshold, per_pixel, hist, deposit, detected,
-                                inelastic_batch, inelastic_wait, overflow_depth, allow_drops)
+                                inelastic_batch, inelastic_wait, overflow_depth, allow_drops, deposit_records)
 
     def run_persistent_list(self, primaries, tags=None, block_size=0, blocks_per_cu=0, stack_depth=0,
                             refill_threshold=0, hist=False, deposit=None, detected=0, inelastic_batch=0,
-                            inelastic_wait=0, overflow_depth=0, allow_drops=False):
+                            inelastic_wait=0, overflow_depth=0, allow_drops=False, deposit_records=None):
         """The persistent transport over explicit primaries (PARTICLE_DTYPE),
         tags default to their indices; per-pixel tallies are not available."""
         prim = np.ascontiguousarray(primaries, L.PARTICLE_DTYPE)
@@ -187,11 +190,11 @@ class Driver:
         self._keep_list = (prim, tags)
         return self._persistent("nbl_driver_run_persistent_list", (ptr(prim), ptr(tags), len(prim)), 0, block_size,
                                 blocks_per_cu, stack_depth, refill_threshold, False, hist, deposit, detected,
-                                inelastic_batch, inelastic_wait, overflow_depth, allow_drops)
+                                inelastic_batch, inelastic_wait, overflow_depth, allow_drops, deposit_records)
 
     def _persistent(self, fn, head, n_px, block_size, blocks_per_cu, stack_depth, refill_threshold, per_pixel,
                     hist, deposit, detected, inelastic_batch=0, inelastic_wait=0, overflow_depth=0,
-                    allow_drops=False):
+                    allow_drops=False, deposit_records=None):
         o = L.PersistentOpts(block_size=block_size, blocks_per_cu=blocks_per_cu, stack_depth=stack_depth,
                              refill_threshold=refill_threshold, inelastic_batch=inelastic_batch,
                              inelastic_wait=inelastic_wait, overflow_depth=overflow_depth,
@@ -214,9 +217,18 @@ class Driver:
             dtags = np.zeros(int(detected), np.uint32)
             det = L.DetectionBuffer(particles=ptr(dparts), tags=ptr(dtags), capacity=int(detected))
             t.detected = C.cast(C.pointer(det), C.c_void_p)
+        drb = rparts = rtags = None
+        if deposit_records is not None:
+            rparts = np.zeros(int(deposit_records), L.DEPOSIT_RECORD_DTYPE)
+            rtags = np.zeros(int(deposit_records), np.uint32)
+            drb = L.DepositBuffer(records=ptr(rparts), tags=ptr(rtags), capacity=int(deposit_records))
+            t.deposits = C.cast(C.pointer(drb), C.c_void_p)
         call(fn, self.h, *head, C.byref(o), C.byref(t))
         out = {f: getattr(t, f) for f, _ in L.Tallies._fields_
-               if f not in ("pixel_se", "pixel_bse", "hist", "deposit", "detected")}
+               if f not in ("pixel_se", "pixel_bse", "hist", "deposit", "detected", "deposits")}
+        if drb is not None:
+            k = min(drb.count, int(deposit_records))
+            out["deposit_records"], out["deposit_records_count"] = (rparts[:k], rtags[:k]), drb.count
         if det is not None:
             k = min(det.count, int(detected))
             out["detected"], out["detected_count"] = (dparts[:k], dtags[:k]), det.count

@@ -123,7 +123,7 @@ def test_cli_reads_hdf5_materials(tmp_path):
 def test_cli_persistent_mode(tmp_path):
     """--persistent 1: the same files through the persistent kernel; the
     records sit on the detector with valid pixels and the yields agree with
-    the frame loop's (same seed, different random streams) within 3 sigma,
+    the frame loop's (same fresh seed, different random streams) within 3 sigma,
     each side's per-primary variance from its records."""
     print("[gpu] nbl_gpu --persistent", flush=True)
     tris = scenes.cube_slab()
@@ -135,7 +135,14 @@ def test_cli_persistent_mode(tmp_path):
     pix = np.stack([np.arange(n) % 100, np.arange(n) // 100], axis=1).astype(np.int32)
     files.write_pri_file(tmp_path / "p.pri", prim, pix)
     synth.write_material_file(synth.SI, tmp_path / "si.nblmat")
-    common = ["--capacity", "200000", "--seed", "7", tmp_path / "g.tri", tmp_path / "p.pri", tmp_path / "si.nblmat"]
+    # The frame loop's outcome is a fixed function of its seed; a fixed seed
+    # would compare every run against the same frame-loop sample, so that a
+    # sample one sigma off the mean fails the 3-sigma bar far more often than
+    # 0.27 %.  A fresh seed per run (printed) keeps the test a two-sample test.
+    seed = int(np.random.SeedSequence().entropy % (1 << 31))
+    print(f"  seed {seed}", flush=True)
+    common = ["--capacity", "200000", "--seed", str(seed), tmp_path / "g.tri", tmp_path / "p.pri",
+              tmp_path / "si.nblmat"]
     out_p, err = _cli("--persistent", "1", *common)
     assert "persistent" in err
     out_f, _ = _cli(*common)

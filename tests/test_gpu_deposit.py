@@ -87,3 +87,45 @@ def test_deposit_grid_errors():
             drv.run_persistent(beam, deposit=dict(shape=(0, 1, 1), lo=LO, hi=HI))
         with pytest.raises(na.NblError):
             drv.run_persistent(beam, deposit=dict(shape=(1, 1, 1), lo=HI, hi=LO))
+
+
+def test_deposit_records_match_the_grid_of_the_same_launch():
+    """Per-event records (cpu_energydep.cpp:184-192: position after the
+    event, energy before, energy lost, tag) from the launch that
+    also fills the grid: binned with the kernel's own float voxel rule they
+    give the grid (sums in double, order aside), the record count is the
+    deposit count and their sum the total; every record loses a positive
+    amount of at most its energy.  A second, count-only call and a call
+    with a small buffer give the counts of their own launches."""
+    print("[gpu] energy-deposition records vs grid", flush=True)
+    tris = scenes.cube_slab()
+    g = na.Geometry(tris); m = na.Materials([synth.SI])
+    n = 2000
+    with Driver(g, m, capacity=1024, seed=21) as drv:
+        beam = raster_scan(np.zeros(1), np.zeros(1), 5.0, 1000.0, n, 0.0, seed=7)
+        t = drv.run_persistent(beam, deposit=dict(shape=SHAPE, lo=LO, hi=HI), deposit_records=6_000_000)
+        assert t["primaries"] == n and t["stopped"] == 0
+        rec, tags = t["deposit_records"]
+        k = t["deposit_records_count"]
+        assert k == t["deposit_count"] == len(rec) > 100 * n
+        assert np.all(tags == 0)                   # a pixel batch tags records with the pixel index
+        assert np.all(rec["deposit"] > 0) and np.all(rec["deposit"] <= rec["energy"])
+        # no electron has more than the beam energy plus the barrier it gains entering the solid
+        barrier = m.prepared[0].desc.barrier
+        assert np.all(rec["energy"] <= np.float32(1000.0 + barrier + 1e-3))
+        assert abs(rec["deposit"].astype(np.float64).sum() - t["deposit_total"]) <= 1e-9 * t["deposit_total"]
+        # the kernel's voxel rule, in float32: floor((p - lo) * (n / (hi - lo)))
+        lo = np.asarray(LO, np.float32); hi = np.asarray(HI, np.float32)
+        inv = np.asarray(SHAPE, np.float32) / (hi - lo)
+        idx = np.floor((rec["pos"] - lo) * inv).astype(np.int64)
+        inside = np.all((idx >= 0) & (idx < np.asarray(SHAPE)), axis=1)
+        flat = (idx[inside, 2] * SHAPE[1] + idx[inside, 1]) * SHAPE[0] + idx[inside, 0]
+        grid = np.bincount(flat, weights=rec["deposit"][inside].astype(np.float64),
+                           minlength=int(np.prod(SHAPE))).reshape(SHAPE[::-1])
+        assert np.allclose(grid, t["deposit"], rtol=1e-9, atol=1e-9 * t["deposit_total"])
+        # records only (no grid), count only, and a buffer smaller than the count
+        t2 = drv.run_persistent(beam, deposit_records=0)
+        assert t2["deposit_records_count"] > 100 * n and len(t2["deposit_records"][0]) == 0
+        t3 = drv.run_persistent(beam, deposit_records=1000)
+        assert t3["deposit_records_count"] > 1000 and len(t3["deposit_records"][0]) == 1000
+        assert np.all(t3["deposit_records"][0]["deposit"] > 0)
