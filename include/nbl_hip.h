@@ -462,6 +462,29 @@ int nbl_driver_run_persistent(nbl_driver d, const nbl_pixel_batch* batch,
 int nbl_driver_run_persistent_list(nbl_driver d, const nbl_particle* primaries, const uint32_t* tags, uint64_t n,
     const nbl_persistent_opts* opts, nbl_tallies* out);
 
+/* HBM secondary-stack space of the allocation-free variant below, per GPU. */
+#define NBL_NOALLOC_STACK_BYTES (64u << 20)
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory while they run (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the transport of
+ * nbl_driver_run_persistent_list that allocates nothing.  Primaries, tags and
+ * the detection records are DEVICE arrays owned by the caller (records in
+ * detection order, the first min(count, detected_capacity) stored; count =
+ * out->detected_se + out->detected_bse).  The kernel's scratch (primary
+ * queues, work counters, histogram, per-wavefront HBM secondary stacks) is one
+ * fixed library array per GPU, NBL_NOALLOC_STACK_BYTES of stack plus 10 KiB,
+ * held until the process ends.  opts->overflow_depth defaults to what fits
+ * (at most 16384); a depth x wavefront count beyond the stack space is
+ * refused (NBL_ERR_INVALID).  out->hist is a host array or NULL;
+ * out->pixel_se, pixel_bse, deposit, detected and deposits must be NULL.  Invalid
+ * primaries are refused as by the list entry point (checked on the device).
+ * Calls that overlap on one GPU share the array and race; run them one at a
+ * time per device.  The driver's stream runs the work; the call returns when
+ * it has finished. */
+int nbl_driver_run_persistent_list_noalloc(nbl_driver d, const nbl_particle* dev_primaries,
+    const uint32_t* dev_tags, uint64_t n, nbl_particle* dev_detected, uint32_t* dev_detected_tags,
+    uint64_t detected_capacity, const nbl_persistent_opts* opts, nbl_tallies* out);
+
 /* ---- file formats (io/load_tri_file.cpp, io/load_pri_file.cpp), host only ---- */
 typedef struct { int32_t x, y; } nbl_int2;     /* int2 of io/load_pri_file.h:10 (pixel index) */
 

@@ -246,6 +246,8 @@ SIGNATURES = {
     "nbl_rng_uniform_batch": [_P, _U32, _U32, _P],
     "nbl_driver_run_persistent": [_P, C.POINTER(PixelBatch), C.POINTER(PersistentOpts), C.POINTER(Tallies)],
     "nbl_driver_run_persistent_list": [_P, _P, _P, _U64, C.POINTER(PersistentOpts), C.POINTER(Tallies)],
+    "nbl_driver_run_persistent_list_noalloc": [_P, _P, _P, _U64, _P, _P, _U64, C.POINTER(PersistentOpts),
+                                               C.POINTER(Tallies)],
     "nbl_tri_file_load": [C.c_char_p, _P, _U32, C.POINTER(_U32)],
     "nbl_pri_file_load": [C.c_char_p, Vec3, Vec3, _F, _P, _P, _U64, C.POINTER(_U64), C.POINTER(PriWarnings)],
     "nbl_pri_sort_morton": [_P, _P, _U64],

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <mutex>
 
 #include "driver.h"
 #include "sampler.h"
@@ -681,6 +682,123 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
     finish_tallies(cnt, hist.data(), ms, L, out);
 }
 
+// ---- the allocation-free variant's scratch -----------------------------------
+// One fixed array per GPU, for callers that cannot allocate during a run:
+// the control words (primary queues, work counters, detection count, the
+// invalid-primary count, the energy histogram) and the per-wavefront HBM
+// secondary stacks.  Host code never passes `scratch` itself to a kernel:
+// its device address comes from hipGetSymbolAddress, once per device.
+constexpr size_t NA_NEXT = 0;                                  // N_QUEUES x QSTRIDE words
+constexpr size_t NA_CNT = NA_NEXT + N_QUEUES * QSTRIDE;        // N_COUNTERS words
+constexpr size_t NA_DET = NA_CNT + 16;                         // detections
+constexpr size_t NA_BAD = NA_DET + 1;                          // invalid primaries
+constexpr size_t NA_HIST = 256;                                // NBL_HIST_BINS words
+constexpr size_t NA_CTRL_WORDS = NA_HIST + NBL_HIST_BINS;      // cleared before every run
+constexpr size_t NA_STACK_WORDS = (size_t)NBL_NOALLOC_STACK_BYTES / 8;
+static_assert(NA_BAD < NA_HIST, "control words overlap the histogram");
+static_assert(N_COUNTERS <= 16, "counter block");
+
+static __device__ __align__(256) uint64_t scratch[NA_CTRL_WORDS + NA_STACK_WORDS];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mutex;
+uint64_t* na_ptr[NA_MAX_DEVICES] = {};
+
+// This device's copy of `scratch` (looked up on the first call per device).
+uint64_t* scratch_on_current_device()
+{
+    int dev = 0;
+    NBL_HIP(hipGetDevice(&dev));
+    require(dev >= 0 && dev < NA_MAX_DEVICES, "nbl_driver_run_persistent_list_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lock(na_mutex);
+    if (!na_ptr[dev]) {
+        void* p = nullptr;
+        NBL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, "nbl_driver_run_persistent_list_noalloc: no device address for the scratch array");
+        na_ptr[dev] = static_cast<uint64_t*>(p);
+    }
+    return na_ptr[dev];
+}
+
+__global__ void k_fill_u64(uint64_t* __restrict__ p, uint32_t n, uint64_t v)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Counts the primaries the list entry point would refuse: a non-finite
+// position, a zero or non-finite direction, a non-finite or non-positive energy.
+__global__ void k_count_invalid(const dparticle* __restrict__ q, uint64_t n, unsigned long long* __restrict__ bad)
+{
+    uint32_t c = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const dparticle p = q[i];
+        const float dd = p.dir.x * p.dir.x + p.dir.y * p.dir.y + p.dir.z * p.dir.z;
+        const bool ok = isfinite(p.pos.x) && isfinite(p.pos.y) && isfinite(p.pos.z) && isfinite(dd) && dd > 0 &&
+                        isfinite(p.e) && p.e > 0;
+        c += ok ? 0u : 1u;
+    }
+    if (c) atomicAdd(bad, (unsigned long long)c);
+}
+
+void run_persistent_noalloc(nbl_driver d, const nbl_particle* primaries, const uint32_t* tags, uint64_t n,
+    nbl_particle* det, uint32_t* det_tags, uint64_t det_cap, const nbl_persistent_opts* opts, nbl_tallies* out)
+{
+    require(n > 0 && primaries && tags, "nbl_driver_run_persistent_list_noalloc: no primaries");
+    require(!out->pixel_se && !out->pixel_bse,
+        "nbl_driver_run_persistent_list_noalloc: per-pixel tallies need a pixel batch (use the detection records)");
+    require(!out->deposit && !out->detected && !out->deposits,
+        "nbl_driver_run_persistent_list_noalloc: out->deposit, out->detected and out->deposits must be NULL (the "
+        "deposit tallies need an allocation; detection records go to the device arrays passed as arguments)");
+    require(det_cap == 0 || (det && det_tags),
+        "nbl_driver_run_persistent_list_noalloc: detected_capacity > 0 needs both record arrays");
+    const launch_shape L0 = resolve_shape(d, opts);
+    launch_shape L = L0;
+    const size_t per_entry = 9 * sizeof(float) * L.n_waves();
+    const size_t max_ovf = (size_t)NBL_NOALLOC_STACK_BYTES / per_entry;
+    if (!L.ovf_given) L.ovf = (uint32_t)std::min<size_t>(L.ovf, max_ovf);
+    if (L.ovf > max_ovf)
+        throw invalid_arg("nbl_driver_run_persistent_list_noalloc: overflow_depth " + std::to_string(L.ovf) +
+            " x " + std::to_string(L.n_waves()) + " wavefronts exceeds the scratch's " +
+            std::to_string(NBL_NOALLOC_STACK_BYTES >> 20) + " MiB of stack: at most " + std::to_string(max_ovf) +
+            " entries per wavefront at this launch shape");
+
+    uint64_t* sp = scratch_on_current_device();
+    hipStream_t s = d->stream;
+    unsigned long long* ctrl = reinterpret_cast<unsigned long long*>(sp);
+    // clear the control words with a kernel (no memset node if the caller captures this stream)
+    hipLaunchKernelGGL(k_fill_u64, dim3(4), dim3(256), 0, s, sp, (uint32_t)NA_CTRL_WORDS, (uint64_t)0);
+    hipLaunchKernelGGL(k_count_invalid, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s,
+        reinterpret_cast<const dparticle*>(primaries), n, ctrl + NA_BAD);
+    NBL_HIP(hipGetLastError());
+    unsigned long long bad = 0;
+    NBL_HIP(hipMemcpyAsync(&bad, ctrl + NA_BAD, sizeof bad, hipMemcpyDeviceToHost, s));
+    NBL_HIP(hipStreamSynchronize(s));
+    require(bad == 0, "nbl_driver_run_persistent_list_noalloc: a primary has a non-finite position, a zero "
+                      "direction or no energy");
+
+    pers_params P = base_params(d, L);
+    P.pixels = nullptr; P.offsets = nullptr; P.n_pixels = 0; P.total = n;
+    P.plist = reinterpret_cast<const dparticle*>(primaries); P.ptags = tags;
+    P.det_on = det_cap ? 1u : 0u;
+    P.det = reinterpret_cast<dparticle*>(det); P.det_tags = det_tags;
+    P.det_count = ctrl + NA_DET; P.det_cap = det_cap;
+    for (uint32_t q = 0; q <= N_QUEUES; ++q) P.qbeg[q] = n * q / N_QUEUES;
+    P.next = ctrl + NA_NEXT; P.hist = ctrl + NA_HIST; P.counters = ctrl + NA_CNT;
+    P.pixel_se = nullptr; P.pixel_bse = nullptr;
+    P.overflow = reinterpret_cast<float*>(sp + NA_CTRL_WORDS);
+    P.dep = nullptr; P.dep_total = nullptr;
+    P.drec_on = 0; P.drec = nullptr; P.drec_tags = nullptr; P.drec_count = nullptr; P.drec_cap = 0;
+
+    const float ms = launch_transport(d, L, P, false);
+
+    uint64_t cnt[N_COUNTERS];
+    unsigned long long hist[NBL_HIST_BINS];
+    NBL_HIP(hipMemcpyAsync(cnt, ctrl + NA_CNT, sizeof cnt, hipMemcpyDeviceToHost, s));
+    if (out->hist) NBL_HIP(hipMemcpyAsync(hist, ctrl + NA_HIST, sizeof hist, hipMemcpyDeviceToHost, s));
+    NBL_HIP(hipStreamSynchronize(s));
+    finish_tallies(cnt, hist, ms, L, out);
+}
+
 }  // namespace
 
 extern "C" {
@@ -755,6 +873,18 @@ int nbl_driver_run_persistent_list(nbl_driver d, const nbl_particle* primaries, 
         primary_source src;
         src.list = primaries; src.tags = tags; src.n_list = n;
         run_persistent(d, src, opts, out);
+    });
+}
+
+int nbl_driver_run_persistent_list_noalloc(nbl_driver d, const nbl_particle* dev_primaries,
+    const uint32_t* dev_tags, uint64_t n, nbl_particle* dev_detected, uint32_t* dev_detected_tags,
+    uint64_t detected_capacity, const nbl_persistent_opts* opts, nbl_tallies* out)
+{
+    return guarded([&] {
+        require(d != nullptr, "nbl_driver_run_persistent_list_noalloc: NULL driver");
+        require(out != nullptr, "nbl_driver_run_persistent_list_noalloc: NULL tallies");
+        run_persistent_noalloc(d, dev_primaries, dev_tags, n, dev_detected, dev_detected_tags, detected_capacity,
+            opts, out);
     });
 }
 

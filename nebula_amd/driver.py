@@ -192,6 +192,30 @@ class Driver:
                                 blocks_per_cu, stack_depth, refill_threshold, False, hist, deposit, detected,
                                 inelastic_batch, inelastic_wait, overflow_depth, allow_drops, deposit_records)
 
+    def run_persistent_list_noalloc(self, dev_primaries, dev_tags, n, dev_detected=None, dev_detected_tags=None,
+                                    detected_capacity=0, hist=False, block_size=0, blocks_per_cu=0, stack_depth=0,
+                                    refill_threshold=0, inelastic_batch=0, inelastic_wait=0, overflow_depth=0,
+                                    allow_drops=False):
+        """The persistent transport over n primaries already in device memory
+        (raw device addresses of PARTICLE_DTYPE records and uint32 tags),
+        through the entry point that allocates nothing: records of detected
+        electrons go to the caller's device arrays (first detected_capacity of
+        them); the count is out["detected_se"] + out["detected_bse"]."""
+        o = L.PersistentOpts(block_size=block_size, blocks_per_cu=blocks_per_cu, stack_depth=stack_depth,
+                             refill_threshold=refill_threshold, inelastic_batch=inelastic_batch,
+                             inelastic_wait=inelastic_wait, overflow_depth=overflow_depth,
+                             allow_drops=int(bool(allow_drops)))
+        t = L.Tallies()
+        h = np.zeros(L.HIST_BINS, np.uint32) if hist else None
+        t.hist = ptr(h)
+        call("nbl_driver_run_persistent_list_noalloc", self.h, dev_primaries, dev_tags, int(n), dev_detected,
+             dev_detected_tags, int(detected_capacity), C.byref(o), C.byref(t))
+        out = {f: getattr(t, f) for f, _ in L.Tallies._fields_
+               if f not in ("pixel_se", "pixel_bse", "hist", "deposit", "detected", "deposits")}
+        if hist:
+            out["hist"] = h
+        return out
+
     def _persistent(self, fn, head, n_px, block_size, blocks_per_cu, stack_depth, refill_threshold, per_pixel,
                     hist, deposit, detected, inelastic_batch=0, inelastic_wait=0, overflow_depth=0,
                     allow_drops=False, deposit_records=None):
