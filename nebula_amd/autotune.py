@@ -34,7 +34,7 @@ DEFAULT_SPACE = {
     "refill_threshold": (1, 4, 8, 32),
     "inelastic_batch": (1, 8),
 }
-PATH_MEMO_LEVELS = 16        # per-lane octree path memo in LDS (lds_path_cache::kLevels)
+PATH_MEMO_WORDS = 19         # per-lane octree path memo in LDS (lds_path_cache::kWords)
 
 
 def _signature(driver, beam, sample):
@@ -63,7 +63,7 @@ def _sample_beam(beam, sample):
 
 def valid(cfg):
     waves = cfg["block_size"] // 64
-    lds = waves * (9 * cfg["stack_depth"] + PATH_MEMO_LEVELS * 64) * 4
+    lds = waves * (9 * cfg["stack_depth"] + PATH_MEMO_WORDS * 64) * 4
     return lds * cfg["blocks_per_cu"] <= 160 * 1024 and cfg["block_size"] * cfg["blocks_per_cu"] <= 2048
 
 

@@ -55,8 +55,8 @@ __global__ __launch_bounds__(256) void k_propagate(dgeom g, const f3* __restrict
     const int32_t* __restrict__ ignore_mat, uint32_t n, float* __restrict__ out_dist,
     uint32_t* __restrict__ out_tri)
 {
-    __shared__ uint32_t memo[4 * lds_path_cache::kLevels * 64];
-    lds_path_cache pc{ memo + (threadIdx.x >> 6) * lds_path_cache::kLevels * 64 + (threadIdx.x & 63), 1ull, 0ull, 0u };
+    __shared__ uint32_t memo[4 * lds_path_cache::kWords * 64];
+    lds_path_cache pc{ memo + (threadIdx.x >> 6) * lds_path_cache::kWords * 64 + (threadIdx.x & 63), 1ull, 0ull, 0u };
     // wave-uniform loop: every lane of a wavefront enters propagate_wave together
     for (uint32_t w0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); w0 < n; w0 += gridDim.x * blockDim.x) {
         const uint32_t i = w0 + (threadIdx.x & 63u);
@@ -117,6 +117,9 @@ int nbl_geometry_create(const nbl_triangle* tris, uint32_t n, nbl_geometry* out)
         v.nodes = g->nodes.p; v.recs = g->recs.p; v.leaves = g->leaves.p; v.ropes = g->ropes.p; v.n_ropes = (uint32_t)std::max<size_t>(g->host.ropes.size(), 6); v.id_rec = g->id_rec.p;
         v.cx = g->host.center[0]; v.cy = g->host.center[1]; v.cz = g->host.center[2];
         v.hx = g->host.halfsize[0]; v.hy = g->host.halfsize[1]; v.hz = g->host.halfsize[2];
+        v.mx = ldexpf(std::fabs(v.cx) + v.hx, -16);
+        v.my = ldexpf(std::fabs(v.cy) + v.hy, -16);
+        v.mz = ldexpf(std::fabs(v.cz) + v.hz, -16);
         v.max_extent = g->host.max_extent;
         *out = g.release();
     });

@@ -117,6 +117,8 @@ struct dgeom {
     float cx, cy, cz;          // root centre
     float hx, hy, hz;          // root half size
     float max_extent;          // 2*|halfsize|, octree.inl:300
+    float mx, my, mz;          // (|centre| + half size) * 2^-16 per axis: more than 128 ulp
+                               // of any cell centre, the leaf-box margin of descend()
 };
 
 // Work counters of a walk: the loads it issues (node_steps: child words,
@@ -240,9 +242,11 @@ NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s
 // the memo's cost centre arithmetic only; the node where the paths part is
 // read once from LDS, and only the levels below it are loaded from HBM.
 // Internal nodes of levels 1..kLevels-1 are kept in LDS, the leaf word in
-// registers.  Centres are always recomputed, so results do not depend on it.
+// registers, and the leaf's centre in LDS after the node words (the leaf-box
+// test of a new walk below).  Centres are otherwise always recomputed.
 struct lds_path_cache {
     static constexpr uint32_t kLevels = 16;
+    static constexpr uint32_t kWords = kLevels + 3;   // per lane: node words, then the leaf centre
     uint32_t* node;      // this lane's column: node[l * 64] = internal node at depth l
     uint64_t loc;        // location code of the memo's leaf (1 = empty)
     uint64_t leaf;       // the leaf's child word
@@ -253,6 +257,16 @@ struct lds_path_cache {
     NBL_HD void store(uint32_t l, uint64_t child)
     {
         if (l < kLevels) node[l * 64] = (uint32_t)child;
+    }
+    NBL_HD void store_centre(f3 c)
+    {
+        float* f = reinterpret_cast<float*>(node + kLevels * 64);
+        f[0] = c.x; f[64] = c.y; f[128] = c.z;
+    }
+    NBL_HD f3 centre() const
+    {
+        const float* f = reinterpret_cast<const float*>(node + kLevels * 64);
+        return { f[0], f[64], f[128] };
     }
     // internal node at depth l < depth of the memo's path
     NBL_HD uint32_t node_at(const dgeom& g, uint32_t l) const
@@ -296,8 +310,28 @@ NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s
         node = (uint32_t)child;
         leaf = (child & LEAF_BIT) != 0;
     } else if (D == 0) {
-        // a new walk (code 1): by position from the root, along the memo
-        // while the octants agree (what the general case below gives for D = 0)
+        // A new walk (code 1).  It usually starts where the lane's last walk
+        // ended, inside the memo's leaf.  The faces of a leaf (centre c, half
+        // size s) are, in exact arithmetic, centre planes of its ancestors;
+        // the float centres the descent compares with differ from the exact
+        // ones by less than 11 ulp of the largest centre (21 levels of one
+        // rounded addition each), so a point inside [c - s + m, c + s - m]
+        // with m = 128 such ulp (the geometry's margin) is on the leaf's side
+        // of every plane on its path, and the descent by position would
+        // return this same leaf: take it without descending.
+        if (M > 0) {
+            const f3 cm = pc->centre();
+            const float sx = ldexpf(g.hx, -(int)M), sy = ldexpf(g.hy, -(int)M), sz = ldexpf(g.hz, -(int)M);
+            if (fabsf(pos.x - cm.x) < sx - g.mx && fabsf(pos.y - cm.y) < sy - g.my &&
+                fabsf(pos.z - cm.z) < sz - g.mz) {
+                location = pc->loc;
+                c = cm;
+                s = { sx, sy, sz };
+                return pc->leaf;
+            }
+        }
+        // otherwise by position from the root, along the memo while the
+        // octants agree (what the general case below gives for D = 0)
         matched = true;
     } else {
         // leading digits the code shares with the memo's leaf
@@ -368,6 +402,7 @@ NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s
     pc->loc = location;
     pc->depth = level;
     pc->leaf = child;
+    pc->store_centre(a);
     c = a;
     s = { h.x + h.x, h.y + h.y, h.z + h.z };
     return child;

@@ -175,14 +175,17 @@ __device__ void make_primary(const pers_params& P, uint64_t i, dparticle& p, uin
 
 // LEARNED: the elastic and inelastic ICDF draws come from the learned
 // sampler's networks (evaluated by the whole wavefront) instead of the tables.
-// The learned variant needs ~150 registers while the networks run, so it is
-// built for 12 wavefronts per CU (3 per SIMD) instead of 16.
+// The learned variant needs ~150 registers while the networks run; it is
+// built for 16 wavefronts per CU all the same (128 VGPRs, 9 of them spilled
+// around the networks: 10 % faster than 3 wavefronts per SIMD without
+// spills).  With the deposit tally too it would spill 19, so that variant is
+// built for 12 wavefronts per CU (3 per SIMD).
 // DEPOSIT: the energy-deposition tally is on (a separate instantiation, so
 // that the plain kernel carries none of its registers).  PENN: the material
 // set's inelastic model is full Penn (its event runs on its own, not merged
 // with the elastic lanes' direction frame).
 template<bool LEARNED, bool DEPOSIT, bool PENN>
-__global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params P)
+__global__ __launch_bounds__(LEARNED && DEPOSIT ? 768 : 1024) void k_persistent(pers_params P)
 {
     extern __shared__ float lds[];
     const uint32_t wave = threadIdx.x >> 6;
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(LEARNED ? 768 : 1024) void k_persistent(pers_params
     const uint32_t cap = P.depth + P.ovf;
     // per-lane octree path memo after all stacks: [wave][level][lane]
     uint32_t* path_lds = reinterpret_cast<uint32_t*>(lds + (size_t)(blockDim.x >> 6) * 9 * P.depth) +
-                         (size_t)wave * lds_path_cache::kLevels * 64 + lane;
+                         (size_t)wave * lds_path_cache::kWords * 64 + lane;
     lds_path_cache pcache{ path_lds, 1ull, 0ull, 0u };
 
     rng_xorwow rng{ rng_seed(P.rng_seed, P.rng_first + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x) };
@@ -458,15 +461,17 @@ struct launch_shape {
     uint64_t n_threads() const { return (uint64_t)cus * per_cu * block; }
 };
 
-launch_shape resolve_shape(nbl_driver d, const nbl_persistent_opts* opts)
+launch_shape resolve_shape(nbl_driver d, const nbl_persistent_opts* opts, bool deposit)
 {
     launch_shape L;
     int dev = 0;
     NBL_HIP(hipGetDevice(&dev));
     NBL_HIP(hipDeviceGetAttribute(&L.cus, hipDeviceAttributeMultiprocessorCount, dev));
     if (opts && opts->block_size) L.block = opts->block_size;
-    // the learned variant holds 3 wavefronts per SIMD (12 per CU), the table variant 4
-    L.per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : (d->sampler ? 3u : 4u);
+    // every variant holds 4 wavefronts per SIMD (16 per CU) but the learned
+    // one with the deposit tally, which holds 3
+    const bool three_waves = d->sampler && deposit;
+    L.per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : (three_waves ? 3u : 4u);
     if (opts && opts->stack_depth) L.depth = opts->stack_depth;
     if (opts && opts->refill_threshold) L.thresh = opts->refill_threshold;
     L.ovf_given = opts && opts->overflow_depth;
@@ -479,13 +484,14 @@ launch_shape resolve_shape(nbl_driver d, const nbl_persistent_opts* opts)
     require(L.ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
     require(L.block % 64 == 0 && L.block >= 64 && L.block <= 1024,
         "nbl_driver_run_persistent: block_size must be 64..1024, multiple of 64");
-    require(!d->sampler || L.block <= 768, "nbl_driver_run_persistent: block_size must be <= 768 with a learned sampler");
+    require(!three_waves || L.block <= 768,
+        "nbl_driver_run_persistent: block_size must be <= 768 with a learned sampler and a deposit tally");
     L.penn = materials_view(d->mats, 0).model == INEL_PENN;
     require(!(L.penn && d->sampler), "nbl_driver_run_persistent: the learned sampler covers Kieft inelastic tables only");
     require(L.per_cu >= 1 && L.per_cu <= 32, "nbl_driver_run_persistent: blocks_per_cu must be 1..32");
     require(L.depth >= 64 && L.depth <= 1024, "nbl_driver_run_persistent: stack_depth must be 64..1024");
     require(L.thresh >= 1 && L.thresh <= 64, "nbl_driver_run_persistent: refill_threshold must be 1..64");
-    L.lds = (size_t)(L.block / 64) * (9 * L.depth + lds_path_cache::kLevels * 64) * sizeof(float);
+    L.lds = (size_t)(L.block / 64) * (9 * L.depth + lds_path_cache::kWords * 64) * sizeof(float);
     require(L.lds <= 160 * 1024, "nbl_driver_run_persistent: stack_depth x waves exceeds the LDS size");
     return L;
 }
@@ -559,7 +565,7 @@ void run_persistent(nbl_driver d, const primary_source& src, const nbl_persisten
     const uint32_t nkp = (uint32_t)src.kp.size();
     const uint32_t np = src.n_tally;
     require(total > 0, "nbl_driver_run_persistent: no primaries in the batch");
-    const launch_shape L = resolve_shape(d, opts);
+    const launch_shape L = resolve_shape(d, opts, out->deposit != nullptr || out->deposits != nullptr);
 
     dbuf<dpixel> dpix(std::max<uint32_t>(nkp, 1));
     dbuf<uint64_t> doff(nkp + 1);
@@ -751,7 +757,7 @@ void run_persistent_noalloc(nbl_driver d, const nbl_particle* primaries, const u
         "deposit tallies need an allocation; detection records go to the device arrays passed as arguments)");
     require(det_cap == 0 || (det && det_tags),
         "nbl_driver_run_persistent_list_noalloc: detected_capacity > 0 needs both record arrays");
-    const launch_shape L0 = resolve_shape(d, opts);
+    const launch_shape L0 = resolve_shape(d, opts, false);
     launch_shape L = L0;
     const size_t per_entry = 9 * sizeof(float) * L.n_waves();
     const size_t max_ovf = (size_t)NBL_NOALLOC_STACK_BYTES / per_entry;

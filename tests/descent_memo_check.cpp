@@ -14,10 +14,16 @@
 // location code, centre and half size as the reference-order descent, the
 // same node-step count, and leave the memo's LDS levels on its code's path.
 //
-// usage: descent_memo_check <scene.tri> <calls> <seed> [cluster x y z r] [nudge]
-//   cluster: half of the far new steps land in the cube of half size r there
-//   nudge:   the reference-order side sees positions on a centre plane one
-//            ulp higher (a planted mistake: the check must then fail)
+// New steps inside the memo's leaf by more than the geometry's margin take
+// the leaf without descending; new steps are also placed within a few
+// hundred ulp of the last leaf's faces, across that margin.
+//
+// usage: descent_memo_check <scene.tri> <calls> <seed> [cluster x y z r] [nudge] [nomargin]
+//   cluster:  half of the far new steps land in the cube of half size r there
+//   nudge:    the reference-order side sees positions on a centre plane one
+//             ulp higher (a planted mistake: the check must then fail)
+//   nomargin: the leaf-box test without its margin (a planted mistake: the
+//             check must then fail on positions next to a leaf's faces)
 #include "octree_build.h"
 
 #include <cmath>
@@ -83,7 +89,7 @@ int main(int argc, char** argv)
     }
     const long calls = atol(argv[2]);
     std::mt19937_64 rng((uint64_t)atoll(argv[3]));
-    bool cluster = false, nudge = false;
+    bool cluster = false, nudge = false, nomargin = false;
     float cx = 0, cy = 0, cz = 0, cr = 0;
     for (int i = 4; i < argc; ++i) {
         if (!strcmp(argv[i], "cluster") && i + 4 < argc) {
@@ -92,6 +98,7 @@ int main(int argc, char** argv)
             cz = strtof(argv[i + 3], nullptr); cr = strtof(argv[i + 4], nullptr);
             i += 4;
         } else if (!strcmp(argv[i], "nudge")) nudge = true;
+        else if (!strcmp(argv[i], "nomargin")) nomargin = true;
     }
 
     if (!check_next_cell(rng, 2000000)) return 1;
@@ -102,17 +109,21 @@ int main(int argc, char** argv)
     g.n_ropes = (uint32_t)o.ropes.size();
     g.cx = o.center[0]; g.cy = o.center[1]; g.cz = o.center[2];
     g.hx = o.halfsize[0]; g.hy = o.halfsize[1]; g.hz = o.halfsize[2];
+    // the leaf-box margin as the library sets it (geometry_view)
+    g.mx = nomargin ? 0.f : ldexpf(fabsf(g.cx) + g.hx, -16);
+    g.my = nomargin ? 0.f : ldexpf(fabsf(g.cy) + g.hy, -16);
+    g.mz = nomargin ? 0.f : ldexpf(fabsf(g.cz) + g.hz, -16);
     const size_t n_words = o.nodes.size();
 
-    std::vector<uint32_t> lds(lds_path_cache::kLevels * 64, 0xFFFFFFFFu);   // one lane's column
+    std::vector<uint32_t> lds(lds_path_cache::kWords * 64, 0xFFFFFFFFu);   // one lane's column
     lds_path_cache pc{ lds.data(), 1ull, 0ull, 0u };
     prop_counters c_ref{}, c_memo{};
     std::uniform_real_distribution<float> U(-1.f, 1.f);
 
-    f3 pos{ g.cx, g.cy, g.cz }, ls{ g.hx, g.hy, g.hz };
+    f3 pos{ g.cx, g.cy, g.cz }, ls{ g.hx, g.hy, g.hz }, lc = pos;
     std::vector<f3> anc{ pos };   // centres of the internal nodes on the last path
     uint64_t last_loc = 1, last_leaf = 0;
-    long n_new = 0, n_rope = 0, n_code = 0, n_plane = 0, max_depth = 0;
+    long n_new = 0, n_rope = 0, n_code = 0, n_plane = 0, n_face = 0, n_box = 0, max_depth = 0;
     for (long it = 0; it < calls; ++it) {
         uint64_t loc = 1;
         uint32_t rope = NO_ROPE;
@@ -122,7 +133,18 @@ int main(int argc, char** argv)
             const int m = (int)(rng() % 8);
             if (m == 0 && cluster && rng() % 2) pos = { cx + cr * U(rng), cy + cr * U(rng), cz + cr * U(rng) };
             else if (m == 0) pos = { g.cx + g.hx * U(rng), g.cy + g.hy * U(rng), g.cz + g.hz * U(rng) };
-            else {
+            else if (m == 7 && pc.depth > 0) {
+                // next to a face of the last leaf: within +-400 ulp of the
+                // largest coordinate, on either side of the margin
+                const uint32_t a = (uint32_t)(rng() % 3);
+                const float sgn = rng() % 2 ? 1.f : -1.f;
+                const float cc = a == 0 ? lc.x : a == 1 ? lc.y : lc.z, hh = a == 0 ? ls.x : a == 1 ? ls.y : ls.z;
+                const float big = (a == 0 ? fabsf(g.cx) + g.hx : a == 1 ? fabsf(g.cy) + g.hy : fabsf(g.cz) + g.hz);
+                const float ulp = ldexpf(1.f, std::ilogb(big) - 23);
+                const float v = cc + sgn * (hh - ulp * (float)((long)(rng() % 801) - 400));
+                if (a == 0) pos.x = v; else if (a == 1) pos.y = v; else pos.z = v;
+                ++n_face;
+            } else {
                 const float sc = m < 4 ? 0.1f : 2.f;
                 pos = { pos.x + sc * ls.x * U(rng), pos.y + sc * ls.y * U(rng), pos.z + sc * ls.z * U(rng) };
             }
@@ -149,6 +171,9 @@ int main(int argc, char** argv)
         n_plane += on_plane;
         f3 pref = pos;
         if (nudge && on_plane) pref.x = nextafterf(pref.x, INFINITY), pref.y = nextafterf(pref.y, INFINITY);
+        // new steps the leaf-box test takes without descending (counted, not checked here)
+        if (kind == 0 && pc.depth > 0 && fabsf(pos.x - lc.x) < ls.x - g.mx && fabsf(pos.y - lc.y) < ls.y - g.my &&
+            fabsf(pos.z - lc.z) < ls.z - g.mz) ++n_box;
         uint64_t l_ref = loc, l_memo = loc;
         f3 c_a{}, s_a{}, c_b{}, s_b{};
         const uint64_t w_ref = descend<true>(g, pref, l_ref, c_a, s_a, &c_ref, (no_path_cache*)nullptr);
@@ -173,7 +198,7 @@ int main(int argc, char** argv)
             if (lds[l * 64] != (uint32_t)n) { printf("call %ld: memo level %u is off the path\n", it, l); return 1; }
         }
         if ((long)pc.depth > max_depth) max_depth = pc.depth;
-        last_loc = l_memo; last_leaf = (uint32_t)w_memo; ls = s_b;
+        last_loc = l_memo; last_leaf = (uint32_t)w_memo; ls = s_b; lc = c_b;
         {
             const uint32_t depth = (uint32_t)(63 - clz64(l_memo)) / 3;
             f3 a{ g.cx, g.cy, g.cz }, h{ 0.5f * g.hx, 0.5f * g.hy, 0.5f * g.hz };
@@ -185,7 +210,8 @@ int main(int argc, char** argv)
         }
         if (rng() % 64 == 0) last_loc = 1;
     }
-    printf("ok calls %ld new %ld rope %ld code %ld on_plane %ld node_steps %u ref_node_steps %u max_depth %ld tree_depth %u\n",
-        calls, n_new, n_rope, n_code, n_plane, c_memo.node_steps, c_ref.node_steps, max_depth, o.max_depth);
+    printf("ok calls %ld new %ld rope %ld code %ld on_plane %ld face %ld leaf_box %ld node_steps %u ref_node_steps %u "
+           "max_depth %ld tree_depth %u\n", calls, n_new, n_rope, n_code, n_plane, n_face, n_box, c_memo.node_steps,
+           c_ref.node_steps, max_depth, o.max_depth);
     return 0;
 }
