@@ -120,6 +120,7 @@ int nbl_geometry_create(const nbl_triangle* tris, uint32_t n, nbl_geometry* out)
         v.mx = ldexpf(std::fabs(v.cx) + v.hx, -16);
         v.my = ldexpf(std::fabs(v.cy) + v.hy, -16);
         v.mz = ldexpf(std::fabs(v.cz) + v.hz, -16);
+        set_derived(v);
         v.max_extent = g->host.max_extent;
         *out = g.release();
     });

@@ -119,6 +119,11 @@ struct dgeom {
     float max_extent;          // 2*|halfsize|, octree.inl:300
     float mx, my, mz;          // (|centre| + half size) * 2^-16 per axis: more than 128 ulp
                                // of any cell centre, the leaf-box margin of descend()
+    // derived once on the host (kernel arguments, so that the device does not
+    // keep them in vector registers): fl(centre -+ half size), the domain
+    // bounds of in_domain, and 0.5 * half size, the first level's offset
+    float lox, loy, loz, hix, hiy, hiz;
+    float h1x, h1y, h1z;
 };
 
 // Work counters of a walk: the loads it issues (node_steps: child words,
@@ -126,10 +131,18 @@ struct dgeom {
 // dropped by a cull costs none); the oracle counts the reference's own work.
 struct prop_counters { uint32_t node_steps, tri_tests; };
 
+// Fills the derived fields of a geometry view from its centre and half size
+// (the same float operations the device would do).
+NBL_HD void set_derived(dgeom& g)
+{
+    g.lox = g.cx - g.hx; g.loy = g.cy - g.hy; g.loz = g.cz - g.hz;
+    g.hix = g.cx + g.hx; g.hiy = g.cy + g.hy; g.hiz = g.cz + g.hz;
+    g.h1x = 0.5f * g.hx; g.h1y = 0.5f * g.hy; g.h1z = 0.5f * g.hz;
+}
+
 NBL_HD bool in_domain(const dgeom& g, f3 p)   // octree.inl:24-30
 {
-    return (p.x > g.cx - g.hx) && (p.x < g.cx + g.hx) && (p.y > g.cy - g.hy) &&
-           (p.y < g.cy + g.hy) && (p.z > g.cz - g.hz) && (p.z < g.cz + g.hz);
+    return (p.x > g.lox) && (p.x < g.hix) && (p.y > g.loy) && (p.y < g.hiy) && (p.z > g.loz) && (p.z < g.hiz);
 }
 
 // triangle::intersect_ray, core/triangle.inl:7-28.
@@ -210,7 +223,7 @@ template<bool COUNT>
 NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s,
     prop_counters* cnt, no_path_cache*, uint32_t = NO_ROPE)
 {
-    f3 a{ g.cx, g.cy, g.cz }, h{ 0.5f * g.hx, 0.5f * g.hy, 0.5f * g.hz };
+    f3 a{ g.cx, g.cy, g.cz }, h{ g.h1x, g.h1y, g.h1z };
     uint32_t node = 0;
     uint64_t child = 0;
     bool leaf = false;
@@ -289,7 +302,7 @@ template<bool COUNT>
 NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s,
     prop_counters* cnt, lds_path_cache* pc, uint32_t rope = NO_ROPE)
 {
-    f3 a{ g.cx, g.cy, g.cz }, h{ 0.5f * g.hx, 0.5f * g.hy, 0.5f * g.hz };
+    f3 a{ g.cx, g.cy, g.cz }, h{ g.h1x, g.h1y, g.h1z };
     const uint32_t D = (uint32_t)(63 - clz64(location)) / 3;   // digits of the code
     const uint32_t M = pc->depth;
     uint32_t level = 0, node = 0;

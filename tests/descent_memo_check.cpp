@@ -113,6 +113,7 @@ int main(int argc, char** argv)
     g.mx = nomargin ? 0.f : ldexpf(fabsf(g.cx) + g.hx, -16);
     g.my = nomargin ? 0.f : ldexpf(fabsf(g.cy) + g.hy, -16);
     g.mz = nomargin ? 0.f : ldexpf(fabsf(g.cz) + g.hz, -16);
+    set_derived(g);
     const size_t n_words = o.nodes.size();
 
     std::vector<uint32_t> lds(lds_path_cache::kWords * 64, 0xFFFFFFFFu);   // one lane's column
