@@ -430,9 +430,13 @@ typedef struct {
 } nbl_tallies;
 
 typedef struct {
-    uint32_t block_size;       /* threads per workgroup (64..1024, multiple of 64); 0 = default */
-    uint32_t blocks_per_cu;    /* resident workgroups per CU; 0 = default          */
-    uint32_t stack_depth;      /* per-wavefront secondary stack entries in LDS; 0 = default (128) */
+    uint32_t block_size;       /* threads per workgroup (64..1024, multiple of 64); 0 = default
+                                  (128; 256 with a learned sampler)                       */
+    uint32_t blocks_per_cu;    /* resident workgroups per CU; 0 = default (10: 5 wavefronts
+                                  per SIMD; 4 with a learned sampler, 3 with it and a
+                                  deposit tally)                                          */
+    uint32_t stack_depth;      /* per-wavefront secondary stack entries in LDS; 0 = default
+                                  (96; 128 with a learned sampler)                        */
     uint32_t refill_threshold; /* refill a wavefront when >= this many lanes idle; 0 = default (4) */
     uint32_t overflow_depth;   /* further per-wavefront stack entries in HBM; 0 = default (16384) */
     uint32_t inelastic_batch;  /* run inelastic events once this many lanes of a wavefront
@@ -463,7 +467,7 @@ int nbl_driver_run_persistent_list(nbl_driver d, const nbl_particle* primaries, 
     const nbl_persistent_opts* opts, nbl_tallies* out);
 
 /* HBM secondary-stack space of the allocation-free variant below, per GPU. */
-#define NBL_NOALLOC_STACK_BYTES (64u << 20)
+#define NBL_NOALLOC_STACK_BYTES (128u << 20)
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory while they run (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc): the transport of

@@ -29,20 +29,24 @@ CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "auto
 
 DEFAULT_SPACE = {
     "block_size": (64, 128, 256),
-    "blocks_per_cu": (2, 4, 8),
-    "stack_depth": (64, 128),
-    "refill_threshold": (1, 4, 8, 32),
+    "blocks_per_cu": (4, 5, 8, 10),
+    "stack_depth": (64, 96, 112),
+    "refill_threshold": (2, 4, 8),
     "inelastic_batch": (1, 8),
 }
-PATH_MEMO_WORDS = 19         # per-lane octree path memo in LDS (lds_path_cache::kWords)
+PATH_MEMO_WORDS = 15         # per-lane octree path memo in LDS (lds_path_cache::kWords)
+MAX_WAVES_PER_CU = 20        # the table kernel is built for 5 wavefronts per SIMD
 
 
-def _signature(driver, beam, sample):
+def _signature(driver, beam, sample, space):
     info = driver.geometry.info()
     px = np.asarray(beam["pixels"])
     h = hashlib.sha1()
     h.update(px.tobytes())
     h.update(str((info["n_triangles_in"], info["n_nodes"], info["n_leaves"], sample)).encode())
+    # the library build and the search space: a rebuilt kernel is tuned afresh
+    h.update(str(sorted(space.items())).encode())
+    h.update(str(os.path.getmtime(L.LIB_PATH)).encode())
     name, cus = _device()
     return f"{name}:{cus}:{h.hexdigest()[:16]}"
 
@@ -64,10 +68,11 @@ def _sample_beam(beam, sample):
 def valid(cfg):
     waves = cfg["block_size"] // 64
     lds = waves * (9 * cfg["stack_depth"] + PATH_MEMO_WORDS * 64) * 4
-    return lds * cfg["blocks_per_cu"] <= 160 * 1024 and cfg["block_size"] * cfg["blocks_per_cu"] <= 2048
+    return (lds * cfg["blocks_per_cu"] <= 160 * 1024 and
+            waves * cfg["blocks_per_cu"] <= MAX_WAVES_PER_CU)
 
 
-LIBRARY_DEFAULT = {"block_size": 256, "blocks_per_cu": 4, "stack_depth": 128, "refill_threshold": 4,
+LIBRARY_DEFAULT = {"block_size": 128, "blocks_per_cu": 10, "stack_depth": 96, "refill_threshold": 4,
                    "inelastic_batch": 8}
 
 
@@ -76,7 +81,7 @@ def tune(driver, beam, sample=200_000, space=None, repeats=1, use_cache=True, lo
     candidate of the space that fits a compute unit runs the sample `repeats`
     times (its best device time counts); results list them all."""
     space = space or DEFAULT_SPACE
-    key = _signature(driver, beam, sample)
+    key = _signature(driver, beam, sample, space)
     cache = {}
     if use_cache and os.path.exists(CACHE):
         try:

@@ -258,7 +258,7 @@ NBL_HD uint64_t descend(const dgeom& g, f3 pos, uint64_t& location, f3& c, f3& s
 // registers, and the leaf's centre in LDS after the node words (the leaf-box
 // test of a new walk below).  Centres are otherwise always recomputed.
 struct lds_path_cache {
-    static constexpr uint32_t kLevels = 16;
+    static constexpr uint32_t kLevels = 12;
     static constexpr uint32_t kWords = kLevels + 3;   // per lane: node words, then the leaf centre
     uint32_t* node;      // this lane's column: node[l * 64] = internal node at depth l
     uint64_t loc;        // location code of the memo's leaf (1 = empty)

@@ -173,19 +173,26 @@ __device__ void make_primary(const pers_params& P, uint64_t i, dparticle& p, uin
     tag = px.tally;
 }
 
+// Occupancy.  The loop is a chain of dependent loads and issue-bound, so
+// more resident wavefronts hide more of it: the table variants are built for
+// 5 wavefronts per SIMD (96 VGPRs; 5 live values spill to scratch around the
+// rare paths), 6.6 % faster than 4 without spills (and 6 per SIMD, 80 VGPRs
+// with 17 spilled, is 14 % slower than 4).  Their LDS (stack + path memo of
+// 20 wavefronts) fits 160 KB for stack depths up to 112.
 // LEARNED: the elastic and inelastic ICDF draws come from the learned
 // sampler's networks (evaluated by the whole wavefront) instead of the tables.
 // The learned variant needs ~150 registers while the networks run; it is
-// built for 16 wavefronts per CU all the same (128 VGPRs, 9 of them spilled
+// built for 4 wavefronts per SIMD all the same (128 VGPRs, 9 of them spilled
 // around the networks: 10 % faster than 3 wavefronts per SIMD without
 // spills).  With the deposit tally too it would spill 19, so that variant is
-// built for 12 wavefronts per CU (3 per SIMD).
+// built for 3 wavefronts per SIMD.
 // DEPOSIT: the energy-deposition tally is on (a separate instantiation, so
 // that the plain kernel carries none of its registers).  PENN: the material
 // set's inelastic model is full Penn (its event runs on its own, not merged
 // with the elastic lanes' direction frame).
 template<bool LEARNED, bool DEPOSIT, bool PENN>
-__global__ __launch_bounds__(LEARNED && DEPOSIT ? 768 : 1024) void k_persistent(pers_params P)
+__global__ __launch_bounds__(LEARNED && DEPOSIT ? 768 : 1024) __attribute__((amdgpu_waves_per_eu(LEARNED ? 1 : 5)))
+void k_persistent(pers_params P)
 {
     extern __shared__ float lds[];
     const uint32_t wave = threadIdx.x >> 6;
@@ -454,7 +461,7 @@ struct primary_source {
 // Launch shape and cadence of a persistent run (nbl_persistent_opts; 0 = default).
 struct launch_shape {
     int cus = 0;
-    uint32_t block = 256, per_cu = 4, depth = 128, thresh = 4, ovf = 16384, ibatch = 8, iwait = 8;
+    uint32_t block = 128, per_cu = 10, depth = 96, thresh = 4, ovf = 16384, ibatch = 8, iwait = 8;
     bool ovf_given = false, allow_drops = false, penn = false;
     size_t lds = 0;
     size_t n_waves() const { return (size_t)cus * per_cu * (block / 64); }
@@ -467,11 +474,13 @@ launch_shape resolve_shape(nbl_driver d, const nbl_persistent_opts* opts, bool d
     int dev = 0;
     NBL_HIP(hipGetDevice(&dev));
     NBL_HIP(hipDeviceGetAttribute(&L.cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if (opts && opts->block_size) L.block = opts->block_size;
-    // every variant holds 4 wavefronts per SIMD (16 per CU) but the learned
-    // one with the deposit tally, which holds 3
+    // defaults: the table variants 128 x 10 (5 wavefronts per SIMD), stack 96;
+    // the learned one 256 x 4 (4 per SIMD), or 256 x 3 with the deposit tally
+    // (3 per SIMD), stack 128
     const bool three_waves = d->sampler && deposit;
-    L.per_cu = opts && opts->blocks_per_cu ? opts->blocks_per_cu : (three_waves ? 3u : 4u);
+    if (d->sampler) { L.block = 256; L.per_cu = three_waves ? 3u : 4u; L.depth = 128; }
+    if (opts && opts->block_size) L.block = opts->block_size;
+    if (opts && opts->blocks_per_cu) L.per_cu = opts->blocks_per_cu;
     if (opts && opts->stack_depth) L.depth = opts->stack_depth;
     if (opts && opts->refill_threshold) L.thresh = opts->refill_threshold;
     L.ovf_given = opts && opts->overflow_depth;

@@ -16,16 +16,20 @@
 namespace nbl {
 
 // Exclusive prefix sum over the 64 lanes (all lanes active); *total = sum.
+// Inclusive scan with DPP lane moves (no LDS, no address registers): row_shr
+// 1, 2, 4, 8 within each row of 16 lanes (a lane with no source adds 0),
+// then row_bcast:15 adds each row's last lane to the next row (rows 1 and
+// 3) and row_bcast:31 adds lane 31 to rows 2 and 3.
 __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& total)
 {
-    const uint32_t lane = lane_id();
     uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d);
-        if (lane >= (uint32_t)d) x += y;
-    }
-    total = __shfl(x, 63);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);   // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);   // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);   // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);   // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);   // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);   // row_bcast:31
+    total = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
     return x - v;
 }
 
