@@ -310,12 +310,15 @@ def main():
     traffic = float(prof["traffic_bytes"]) if prof else None
     issue = None
     if prof and prof.get("valu_insts") and prof.get("gpu_cycles"):
-        # every VALU instruction occupies its SIMD for 4 cycles (64 lanes on 16-wide units)
-        busy = 4 * prof["valu_insts"] / N_SIMDS
-        issue = {"what": "vector ALU issue", "valu_insts_per_launch": prof["valu_insts"],
-                 "valu_busy_cycles_per_simd": busy, "launch_cycles": prof["gpu_cycles"],
-                 "frac": busy / prof["gpu_cycles"], "valu_insts_per_event": prof["valu_insts"] / per_launch["events"],
-                 "source": prof_src}
+        # vector instructions issued per SIMD per cycle of the launch (a SIMD
+        # issues at most one per cycle; at 4 cycles per wave64 instruction
+        # the profiled launch would not fit its own duration, so gfx950
+        # retires many in fewer, and this issue rate is what is measured)
+        per_simd = prof["valu_insts"] / N_SIMDS
+        issue = {"what": "vector ALU instruction issue per SIMD per cycle", "valu_insts_per_launch": prof["valu_insts"],
+                 "valu_insts_per_simd": per_simd, "launch_cycles": prof["gpu_cycles"],
+                 "frac": per_simd / prof["gpu_cycles"], "cycles_per_valu_inst": prof["gpu_cycles"] / per_simd,
+                 "valu_insts_per_event": prof["valu_insts"] / per_launch["events"], "source": prof_src}
 
     cpu = None
     if not args.no_cpu:
