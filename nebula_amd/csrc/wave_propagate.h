@@ -33,6 +33,37 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t& tot
     return x - v;
 }
 
+// Inclusive maximum over lanes 0..lane (all lanes active), with the lane
+// moves of wave_exclusive_sum; a lane with no source keeps its value.
+__device__ __forceinline__ uint32_t wave_inclusive_max(uint32_t x)
+{
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false));
+    return x;
+}
+
+// Owner of candidate base + lane: the last lane whose offset is <= it (the
+// lane found has count > 0 whenever base + lane < total).  Every lane sends
+// its index to the lane at (off - base), or to lane 0 when that is outside
+// the round; a forward permute keeps the highest sender per lane, and lanes
+// with count 0 share their offset with a higher lane, so lane s receives the
+// last lane whose range starts at base + s (0 if none).  Lane 0 takes the
+// owner of item base from a ballot (the last started range), and a running
+// maximum gives every lane its owner.
+__device__ __forceinline__ uint32_t pooled_owner(uint32_t off, uint32_t count, uint32_t base)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t slot = off - base < 64u ? off - base : 0u;
+    uint32_t mark = (uint32_t)__builtin_amdgcn_ds_permute((int)(slot * 4u), (int)lane);
+    const uint64_t started = __ballot(count > 0 && off <= base);
+    if (lane == 0) mark = 63u - (uint32_t)__builtin_clzll(started | 1ull);
+    return wave_inclusive_max(mark);
+}
+
 // A lane's walk between cells (the state octree::propagate keeps across
 // iterations of its cell loop, octree.inl:32-190).
 struct walk_state {
@@ -81,13 +112,8 @@ __device__ __forceinline__ bool walk_cell(bool run, const dgeom& g, walk_state& 
     // candidates [off, off + count) of every lane, 64 per round
     for (uint32_t base = 0; base < total; base += 64) {
         const uint32_t item = base + lane;
-        // owner: the last lane whose offset is <= item (offsets ascend; the
-        // lane found has count > 0 whenever item < total)
-        uint32_t owner = 0;
-#pragma unroll
-        for (uint32_t step = 32; step > 0; step >>= 1)
-            if ((uint32_t)__shfl(off, (int)(owner + step)) <= item) owner += step;
-        const uint32_t k = __shfl(first, (int)owner) + (item - __shfl(off, (int)owner));
+        const uint32_t owner = pooled_owner(off, count, base);
+        const uint32_t k = __shfl(first - off, (int)owner) + item;
         const f3 op{ __shfl(w.pos.x, (int)owner), __shfl(w.pos.y, (int)owner), __shfl(w.pos.z, (int)owner) };
         const f3 od{ __shfl(dir.x, (int)owner), __shfl(dir.y, (int)owner), __shfl(dir.z, (int)owner) };
         const uint32_t oit = __shfl(ignore_tri, (int)owner);
