@@ -537,7 +537,7 @@ typedef struct nbl_sampler_s* nbl_sampler;
 #define NBL_SAMPLER_PARAMS 9988
 typedef struct {
     uint64_t samples;        /* training samples per network; 0 = 1e8            */
-    uint32_t batch;          /* samples per step (multiple of 256); 0 = 16384     */
+    uint32_t batch;          /* samples per step (multiple of 256); 0 = 4096      */
     float lr;                /* Adam learning rate (cosine decay); 0 = 4e-3       */
     float e_min, e_max;      /* kinetic-energy range of the samples, eV; 0 = the tables' */
     uint64_t seed;

@@ -217,7 +217,7 @@ class Sampler:
     KINDS = {L.SAMPLER_ELASTIC: L.TAB_ELASTIC_ICDF, L.SAMPLER_INELASTIC: L.TAB_INELASTIC_W0_ICDF}
     KIND_MASKS = {"both": 3, "elastic": 1 << L.SAMPLER_ELASTIC, "inelastic": 1 << L.SAMPLER_INELASTIC}
 
-    def __init__(self, materials: Materials, samples=100_000_000, batch=16384, lr=4e-3, e_min=0.0, e_max=0.0,
+    def __init__(self, materials: Materials, samples=100_000_000, batch=4096, lr=4e-3, e_min=0.0, e_max=0.0,
                  seed=1, kinds="both"):
         self.materials = materials              # the tables must outlive the sampler
         self.kinds = kinds

@@ -565,6 +565,11 @@ __device__ __forceinline__ float propagate(const dgeom& g, f3 start, f3 dir, flo
 // table_2D.inl:73-104).  2D storage (i, j) -> d[i*h + j], x = energy axis.
 // ---------------------------------------------------------------------------
 struct dtab1 { const float* d; int n; float x_min, x_step; };
+// Table data lives in global memory; loads through this type are global
+// loads (a generic pointer read from a descriptor would give flat loads,
+// which also wait on the LDS and scalar-load counter).
+typedef __attribute__((address_space(1))) const float gfloat;
+__device__ __forceinline__ gfloat* gptr(const float* p) { return (gfloat*)p; }
 struct dtab2 { const float* d; int w, h; float x_min, x_step, y_min, y_step; };
 
 NBL_HD int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
@@ -577,7 +582,7 @@ __device__ __forceinline__ float tab1_get(const dtab1& t, float x)
 {
     const float xi = (x - t.x_min) * t.x_step;
     const int lo = clampi((int)floorf(xi), 0, t.n - 2);
-    const float v0 = t.d[lo], v1 = t.d[lo + 1];
+    const float v0 = gptr(t.d)[lo], v1 = gptr(t.d)[lo + 1];
     const float f = xi - (float)lo;
     return (1 - f) * v0 + f * v1;
 }
@@ -587,8 +592,8 @@ __device__ __forceinline__ float tab2_get(const dtab2& t, float x, float y)
     const float yi = (y - t.y_min) * t.y_step;
     const int lx = clampi((int)floorf(xi), 0, t.w - 2);
     const int ly = clampi((int)floorf(yi), 0, t.h - 2);
-    const float* r0 = t.d + (size_t)lx * t.h + ly;
-    const float* r1 = r0 + t.h;
+    gfloat* r0 = gptr(t.d) + (size_t)lx * t.h + ly;
+    gfloat* r1 = r0 + t.h;
     const float v00 = r0[0], v01 = r0[1], v10 = r1[0], v11 = r1[1];
     const float fx = xi - (float)lx, fy = yi - (float)ly;
     return (1 - fx) * (1 - fy) * v00 + fx * (1 - fy) * v10 + (1 - fx) * fy * v01 + fx * fy * v11;
@@ -599,7 +604,7 @@ __device__ __forceinline__ float tab2_rounddown(const dtab2& t, float x, float y
     const float yi = (y - t.y_min) * t.y_step;
     const int lx = clampi((int)floorf(xi), 0, t.w - 1);
     const int ly = clampi((int)floorf(yi), 0, t.h - 1);
-    return t.d[(size_t)lx * t.h + ly];
+    return gptr(t.d)[(size_t)lx * t.h + ly];
 }
 
 // table_3D::get / get_rounddown (table_3D.inl:78-120); storage (i, j, k) ->
@@ -608,7 +613,7 @@ struct dtab3 { const float* d; int w, h, dp; float x_min, x_step, y_min, y_step,
 
 __device__ __forceinline__ float tab3_at(const dtab3& t, int i, int j, int k)
 {
-    return t.d[((size_t)i * t.h + j) * t.dp + k];
+    return gptr(t.d)[((size_t)i * t.h + j) * t.dp + k];
 }
 __device__ __forceinline__ float tab3_get(const dtab3& t, float x, float y, float z)
 {

@@ -100,16 +100,23 @@ struct pers_params {
 };
 
 // One logical stack per wavefront: entries [0, D) in LDS, [D, D + G) in HBM.
-// Fields are stored SoA: x y z dx dy dz e mat tag.
+// Fields are stored SoA: x y z dx dy dz e mat tag.  The LDS part is typed as
+// LDS (address space 3), so that its accesses are LDS instructions: through
+// one generic pointer for both parts they would be flat accesses, which wait
+// on the vector-memory and the LDS counters alike.
+typedef __attribute__((address_space(3))) float lds_float;
+
 struct wave_stack {
-    float* lds; float* hbm; uint32_t D, G;
-    static __device__ __forceinline__ void store(float* b, size_t s, const dparticle& q, int mat, uint32_t tag)
+    lds_float* lds; float* hbm; uint32_t D, G;
+    template<typename PTR>
+    static __device__ __forceinline__ void store(PTR b, size_t s, const dparticle& q, int mat, uint32_t tag)
     {
         b[0] = q.pos.x; b[s] = q.pos.y; b[2 * s] = q.pos.z;
         b[3 * s] = q.dir.x; b[4 * s] = q.dir.y; b[5 * s] = q.dir.z;
         b[6 * s] = q.e; b[7 * s] = __int_as_float(mat); b[8 * s] = __uint_as_float(tag);
     }
-    static __device__ __forceinline__ void load(const float* b, size_t s, dparticle& q, int& mat, uint32_t& tag)
+    template<typename PTR>
+    static __device__ __forceinline__ void load(PTR b, size_t s, dparticle& q, int& mat, uint32_t& tag)
     {
         q.pos = { b[0], b[s], b[2 * s] };
         q.dir = { b[3 * s], b[4 * s], b[5 * s] };
@@ -198,7 +205,8 @@ void k_persistent(pers_params P)
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
     const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + wave;
-    const wave_stack st{ lds + (size_t)wave * 9 * P.depth, P.overflow + (size_t)gwave * 9 * P.ovf, P.depth, P.ovf };
+    const wave_stack st{ (lds_float*)lds + (size_t)wave * 9 * P.depth, P.overflow + (size_t)gwave * 9 * P.ovf,
+                         P.depth, P.ovf };
     const uint32_t cap = P.depth + P.ovf;
     // per-lane octree path memo after all stacks: [wave][level][lane]
     uint32_t* path_lds = reinterpret_cast<uint32_t*>(lds + (size_t)(blockDim.x >> 6) * 9 * P.depth) +
@@ -272,19 +280,25 @@ void k_persistent(pers_params P)
         uint8_t ev = EV_NONE;
         bool started = false, capped = false, init = false;
         if (alive && !walking) {
-            if (pend) {
-                ev = EV_INELASTIC;
-            } else if (++steps > STEP_CAP) {
-                alive = false; capped = true;
+            if (pend) ev = EV_INELASTIC;
+            else if (++steps > STEP_CAP) { alive = false; capped = true; }
+            else init = true;
+        }
+        {
+            // the sampling lanes' material, read once for the wavefront
+            // when they share one
+            const int mu = wave_material(init && mat >= 0, mat);
+            lane_counters unused;
+            bool ok = false;
+            if (mu >= 0) {
+                const dmat m = load_material(const_mats(P.mats), mu);
+                if (init) ok = init_sample_in<false>(P.g, m, mat >= 0, P.min_e, P.max_e, p, rng, unused, dist, type);
             } else {
-                init = true;
-                lane_counters unused;
-                if (init_sample<false>(P.g, P.mats, P.min_e, P.max_e, p, mat, rng, unused, dist, type)) {
-                    started = walking = true;
-                    w = walk_begin(p.pos, dist);
-                } else {
-                    alive = false;
-                }
+                if (init) ok = init_sample<false>(P.g, P.mats, P.min_e, P.max_e, p, mat, rng, unused, dist, type);
+            }
+            if (init) {
+                if (ok) { started = walking = true; w = walk_begin(p.pos, dist); }
+                else alive = false;
             }
         }
         n_killed += __popcll(__ballot(capped));
@@ -360,14 +374,26 @@ void k_persistent(pers_params P)
                     last = NO_TRI;
                     rot = scatter_begin_v(P.mats[mat], p, rng, el, v, ss);
                 }
-            } else if (el || (go && !PENN)) {
-                pend = false;
-                last = NO_TRI;
-                rot = scatter_begin(P.mats[mat], p, rng, el, ss);
             }
-            f3 nd{ 0, 0, 0 };
-            if (rot) nd = scatter_frame(p, rng);
-            if (rot) made = scatter_end(P.mats[mat], p, rng, ss, nd, &q);
+            // Kieft events from the tables: the scattering lanes' material is
+            // read once for the wavefront when they share one
+            auto kieft = [&](const dmat& m) {
+                if (!LEARNED && (el || (go && !PENN))) {
+                    pend = false;
+                    last = NO_TRI;
+                    rot = scatter_begin(m, p, rng, el, ss);
+                }
+                f3 nd{ 0, 0, 0 };
+                if (rot) nd = scatter_frame(p, rng);
+                if (rot) made = scatter_end(m, p, rng, ss, nd, &q);
+            };
+            const int mu = LEARNED || PENN ? -1 : wave_material(el || go, mat);
+            if (mu >= 0) {
+                const dmat m = load_material(const_mats(P.mats), mu);
+                kieft(m);
+            } else {
+                kieft(P.mats[mat >= 0 ? mat : 0]);
+            }
             if (PENN && go) {
                 pend = false;
                 last = NO_TRI;

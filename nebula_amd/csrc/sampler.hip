@@ -409,7 +409,9 @@ void train_network(nbl_sampler_s& S, int kind, const nbl_sampler_opts& o, float 
         a.y_scale = (float)std::sqrt(var);
     }
     const uint64_t steps = std::max<uint64_t>(1, o.samples / B);
-    const uint32_t chunk = 1024, nchunk = (B + chunk - 1) / chunk;
+    // weight gradients: one block per 128 samples (at least 32 blocks for
+    // the default batch; the step time is set by these reductions' depth)
+    const uint32_t chunk = 128, nchunk = (B + chunk - 1) / chunk;
     double last = 0;
     for (uint64_t st = 0; st < steps; ++st) {
         const bool probe = st + 1 == steps || (st % 64) == 63;
@@ -460,7 +462,7 @@ int nbl_sampler_create(nbl_materials m, const nbl_sampler_opts* opts, nbl_sample
         require(m && out, "nbl_sampler_create: NULL argument");
         *out = nullptr;
         nbl_sampler_opts o{};
-        o.samples = 100000000ull; o.batch = 16384; o.lr = 4e-3f; o.e_min = 0; o.e_max = 0; o.seed = 1;
+        o.samples = 100000000ull; o.batch = 4096; o.lr = 4e-3f; o.e_min = 0; o.e_max = 0; o.seed = 1;
         if (opts) {
             if (opts->samples) o.samples = opts->samples;
             if (opts->batch) o.batch = opts->batch;
