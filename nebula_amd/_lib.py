@@ -169,7 +169,7 @@ class PersistentOpts(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("stack_depth", C.c_uint32), ("refill_threshold", C.c_uint32),
                 ("overflow_depth", C.c_uint32), ("inelastic_batch", C.c_uint32),
-                ("inelastic_wait", C.c_uint32), ("allow_drops", C.c_uint32)]
+                ("inelastic_wait", C.c_uint32), ("allow_drops", C.c_uint32), ("count_work", C.c_uint32)]
 
 
 class Int2(C.Structure):

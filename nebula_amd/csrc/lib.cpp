@@ -18,11 +18,24 @@ void ensure_device()
 }  // namespace nbl
 
 namespace {
-// Plain streaming copy, 16 bytes per lane per iteration.
-__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
+// Plain streaming copy: four independent 16-byte loads per lane in flight,
+// then their stores, with non-temporal hints (the data is touched once).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy(const u32x4* __restrict__ src, u32x4* __restrict__ dst, size_t n)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const u32x4 a = __builtin_nontemporal_load(src + i);
+        const u32x4 b = __builtin_nontemporal_load(src + i + stride);
+        const u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+        const u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+        __builtin_nontemporal_store(a, dst + i);
+        __builtin_nontemporal_store(b, dst + i + stride);
+        __builtin_nontemporal_store(c, dst + i + 2 * stride);
+        __builtin_nontemporal_store(d, dst + i + 3 * stride);
+    }
+    for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 }  // namespace
 
@@ -34,7 +47,7 @@ int nbl_bench_copy(uint64_t bytes, int32_t reps, double* ms_per_copy)
         nbl::require(bytes >= 16 && bytes % 16 == 0 && reps >= 1, "nbl_bench_copy: bytes must be a multiple of 16, reps >= 1");
         nbl::ensure_device();
         const size_t n = bytes / 16;
-        nbl::dbuf<uint4> a(n), b(n);
+        nbl::dbuf<u32x4> a(n), b(n);
         NBL_HIP(hipMemset(a.p, 1, bytes));
         hipEvent_t e0, e1;
         NBL_HIP(hipEventCreate(&e0)); NBL_HIP(hipEventCreate(&e1));

@@ -196,8 +196,15 @@ __device__ void make_primary(const pers_params& P, uint64_t i, dparticle& p, uin
 // DEPOSIT: the energy-deposition tally is on (a separate instantiation, so
 // that the plain kernel carries none of its registers).  PENN: the material
 // set's inelastic model is full Penn (its event runs on its own, not merged
-// with the elastic lanes' direction frame).
-template<bool LEARNED, bool DEPOSIT, bool PENN>
+// with the elastic lanes' direction frame).  WORK: the work counters
+// (events, table lookups, node steps, triangle tests, boundary events,
+// secondaries, waits) are kept; without them a wavefront counts only
+// primaries, detections and stopped electrons.  The counters are a dozen
+// wave-uniform sums and two per-lane ones updated every loop iteration, and
+// cost about 4 % of the kernel time (more scalar registers than the loop
+// has, so they live in vector-register lanes), so runs that do not report
+// them skip them.
+template<bool LEARNED, bool DEPOSIT, bool PENN, bool WORK>
 __global__ __launch_bounds__(LEARNED && DEPOSIT ? 768 : 1024) __attribute__((amdgpu_waves_per_eu(LEARNED ? 1 : 5)))
 void k_persistent(pers_params P)
 {
@@ -302,13 +309,15 @@ void k_persistent(pers_params P)
             }
         }
         n_killed += __popcll(__ballot(capped));
-        n_events += __popcll(__ballot(init));
-        n_l1d += 2 * __popcll(__ballot(started && mat >= 0));
+        if (WORK) {
+            n_events += __popcll(__ballot(init));
+            n_l1d += 2 * __popcll(__ballot(started && mat >= 0));
+        }
         {
             // all lanes enter the cell step; lanes not walking help test triangles
             uint32_t hit = NO_TRI;
             float d = 0;
-            if (walk_cell<true>(walking, P.g, w, p.dir, dist, last, mat, &hit, &d, &walked, &pcache)) {
+            if (walk_cell<WORK>(walking, P.g, w, p.dir, dist, last, mat, &hit, &d, &walked, &pcache)) {
                 walking = false;
                 ev = init_move(p, dist, d, hit, type, last);
             }
@@ -326,9 +335,9 @@ void k_persistent(pers_params P)
         const bool stuck = room == 0 && wm == am;
         const bool go = want && batch && (stuck || rank_below(wm) < room);
         if (want && !go) pend = true;
-        n_pend += __popcll(__ballot(want && batch && !go));
-        n_bound += __popcll(__ballot(alive && ev == EV_INTERSECT));
-        {
+        if (WORK) {
+            n_pend += __popcll(__ballot(want && batch && !go));
+            n_bound += __popcll(__ballot(alive && ev == EV_INTERSECT));
             const uint32_t n_el = __popcll(__ballot(alive && ev == EV_ELASTIC)), n_in = __popcll(__ballot(go));
             n_l2d += n_el + n_in;
             n_rd += n_in;
@@ -404,7 +413,7 @@ void k_persistent(pers_params P)
                 const float de = (el || go) ? e_before - p.e : 0.0f;
                 const bool dep = de != 0.0f;
                 const uint64_t dm = __ballot(dep);
-                n_dep += __popcll(dm);
+                if (WORK) n_dep += __popcll(dm);
                 if (dep) {
                     dep_sum += de;
                     const int ix = (int)floorf((p.pos.x - P.dlo.x) * P.dinv.x);
@@ -448,7 +457,7 @@ void k_persistent(pers_params P)
             const uint32_t j = sp + rank_below(mm);
             if (j < cap) st.put(j, q, mat, tag);
         }
-        n_sec += n_made;
+        if (WORK) n_sec += n_made;
         n_killed += sp + n_made > cap ? sp + n_made - cap : 0u;   // dropped: no room left
         sp = min(sp + n_made, cap);
         wave_sync();
@@ -489,6 +498,7 @@ struct launch_shape {
     int cus = 0;
     uint32_t block = 128, per_cu = 10, depth = 96, thresh = 4, ovf = 16384, ibatch = 8, iwait = 8;
     bool ovf_given = false, allow_drops = false, penn = false;
+    bool work = false;                  // keep the work counters (the learned and deposit kernels always do)
     size_t lds = 0;
     size_t n_waves() const { return (size_t)cus * per_cu * (block / 64); }
     uint64_t n_threads() const { return (uint64_t)cus * per_cu * block; }
@@ -514,6 +524,7 @@ launch_shape resolve_shape(nbl_driver d, const nbl_persistent_opts* opts, bool d
     if (opts && opts->inelastic_batch) L.ibatch = opts->inelastic_batch;
     if (opts && opts->inelastic_wait) L.iwait = opts->inelastic_wait;
     L.allow_drops = opts && opts->allow_drops;
+    L.work = (opts && opts->count_work) || d->sampler || deposit;
     require(L.ibatch <= 64, "nbl_driver_run_persistent: inelastic_batch must be 1..64");
     require(L.iwait <= 1024, "nbl_driver_run_persistent: inelastic_wait must be <= 1024");
     require(L.ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
@@ -559,14 +570,16 @@ float launch_transport(nbl_driver d, const launch_shape& L, const pers_params& P
     NBL_HIP(hipEventRecord(d->ev_start, s));
     const dim3 grid((unsigned)(L.cus * L.per_cu)), threads_per_block(L.block);
     if (d->sampler) {
-        if (deposit) hipLaunchKernelGGL((k_persistent<true, true, false>), grid, threads_per_block, L.lds, s, P);
-        else hipLaunchKernelGGL((k_persistent<true, false, false>), grid, threads_per_block, L.lds, s, P);
+        if (deposit) hipLaunchKernelGGL((k_persistent<true, true, false, true>), grid, threads_per_block, L.lds, s, P);
+        else hipLaunchKernelGGL((k_persistent<true, false, false, true>), grid, threads_per_block, L.lds, s, P);
     } else if (L.penn) {
-        if (deposit) hipLaunchKernelGGL((k_persistent<false, true, true>), grid, threads_per_block, L.lds, s, P);
-        else hipLaunchKernelGGL((k_persistent<false, false, true>), grid, threads_per_block, L.lds, s, P);
+        if (deposit) hipLaunchKernelGGL((k_persistent<false, true, true, true>), grid, threads_per_block, L.lds, s, P);
+        else if (L.work) hipLaunchKernelGGL((k_persistent<false, false, true, true>), grid, threads_per_block, L.lds, s, P);
+        else hipLaunchKernelGGL((k_persistent<false, false, true, false>), grid, threads_per_block, L.lds, s, P);
     } else {
-        if (deposit) hipLaunchKernelGGL((k_persistent<false, true, false>), grid, threads_per_block, L.lds, s, P);
-        else hipLaunchKernelGGL((k_persistent<false, false, false>), grid, threads_per_block, L.lds, s, P);
+        if (deposit) hipLaunchKernelGGL((k_persistent<false, true, false, true>), grid, threads_per_block, L.lds, s, P);
+        else if (L.work) hipLaunchKernelGGL((k_persistent<false, false, false, true>), grid, threads_per_block, L.lds, s, P);
+        else hipLaunchKernelGGL((k_persistent<false, false, false, false>), grid, threads_per_block, L.lds, s, P);
     }
     NBL_HIP(hipGetLastError());
     NBL_HIP(hipEventRecord(d->ev_stop, s));

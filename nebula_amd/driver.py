@@ -159,7 +159,7 @@ class Driver:
 
     def run_persistent(self, beam, block_size=0, blocks_per_cu=0, stack_depth=0, refill_threshold=0,
                        per_pixel=False, hist=False, deposit=None, detected=0, inelastic_batch=0, inelastic_wait=0,
-                       overflow_depth=0, allow_drops=False, deposit_records=None):
+                       overflow_depth=0, allow_drops=False, deposit_records=None, count_work=False):
         """One persistent launch over the pixel batch `beam`.  deposit =
         dict(shape=(nx, ny, nz), lo=(x, y, z), hi=(x, y, z)) turns on the
         energy-deposition tally: out["deposit"] is the [nz, ny, nx] grid (eV),
@@ -170,6 +170,8 @@ class Driver:
         (None: off; 0: count only): out["deposit_records"] = (records of
         DEPOSIT_RECORD_DTYPE, tags), out["deposit_records_count"].
         inelastic_batch, inelastic_wait, overflow_depth: nbl_persistent_opts (0 = defaults).
+        count_work: fill the work counters (events, lookups, node steps, ...;
+        nbl_persistent_opts.count_work), else they are 0.
         A run that stops any electron (a secondary with no stack room left, or
         the step cap) raises NblError unless allow_drops; then out["stopped"]
         counts them."""
@@ -178,11 +180,13 @@ class Driver:
         b = L.PixelBatch(pixels=ptr(px), n_pixels=len(px), dir=L.Vec3(*d), seed=int(beam.get("seed", 0)))
         return self._persistent("nbl_driver_run_persistent", (C.byref(b),), len(px), block_size, blocks_per_cu,
                                 stack_depth, refill_threshold, per_pixel, hist, deposit, detected,
-                                inelastic_batch, inelastic_wait, overflow_depth, allow_drops, deposit_records)
+                                inelastic_batch, inelastic_wait, overflow_depth, allow_drops, deposit_records,
+                                count_work)
 
     def run_persistent_list(self, primaries, tags=None, block_size=0, blocks_per_cu=0, stack_depth=0,
                             refill_threshold=0, hist=False, deposit=None, detected=0, inelastic_batch=0,
-                            inelastic_wait=0, overflow_depth=0, allow_drops=False, deposit_records=None):
+                            inelastic_wait=0, overflow_depth=0, allow_drops=False, deposit_records=None,
+                            count_work=False):
         """The persistent transport over explicit primaries (PARTICLE_DTYPE),
         tags default to their indices; per-pixel tallies are not available."""
         prim = np.ascontiguousarray(primaries, L.PARTICLE_DTYPE)
@@ -190,12 +194,13 @@ class Driver:
         self._keep_list = (prim, tags)
         return self._persistent("nbl_driver_run_persistent_list", (ptr(prim), ptr(tags), len(prim)), 0, block_size,
                                 blocks_per_cu, stack_depth, refill_threshold, False, hist, deposit, detected,
-                                inelastic_batch, inelastic_wait, overflow_depth, allow_drops, deposit_records)
+                                inelastic_batch, inelastic_wait, overflow_depth, allow_drops, deposit_records,
+                                count_work)
 
     def run_persistent_list_noalloc(self, dev_primaries, dev_tags, n, dev_detected=None, dev_detected_tags=None,
                                     detected_capacity=0, hist=False, block_size=0, blocks_per_cu=0, stack_depth=0,
                                     refill_threshold=0, inelastic_batch=0, inelastic_wait=0, overflow_depth=0,
-                                    allow_drops=False):
+                                    allow_drops=False, count_work=False):
         """The persistent transport over n primaries already in device memory
         (raw device addresses of PARTICLE_DTYPE records and uint32 tags),
         through the entry point that allocates nothing: records of detected
@@ -204,7 +209,7 @@ class Driver:
         o = L.PersistentOpts(block_size=block_size, blocks_per_cu=blocks_per_cu, stack_depth=stack_depth,
                              refill_threshold=refill_threshold, inelastic_batch=inelastic_batch,
                              inelastic_wait=inelastic_wait, overflow_depth=overflow_depth,
-                             allow_drops=int(bool(allow_drops)))
+                             allow_drops=int(bool(allow_drops)), count_work=int(bool(count_work)))
         t = L.Tallies()
         h = np.zeros(L.HIST_BINS, np.uint32) if hist else None
         t.hist = ptr(h)
@@ -218,11 +223,11 @@ class Driver:
 
     def _persistent(self, fn, head, n_px, block_size, blocks_per_cu, stack_depth, refill_threshold, per_pixel,
                     hist, deposit, detected, inelastic_batch=0, inelastic_wait=0, overflow_depth=0,
-                    allow_drops=False, deposit_records=None):
+                    allow_drops=False, deposit_records=None, count_work=False):
         o = L.PersistentOpts(block_size=block_size, blocks_per_cu=blocks_per_cu, stack_depth=stack_depth,
                              refill_threshold=refill_threshold, inelastic_batch=inelastic_batch,
                              inelastic_wait=inelastic_wait, overflow_depth=overflow_depth,
-                             allow_drops=int(bool(allow_drops)))
+                             allow_drops=int(bool(allow_drops)), count_work=int(bool(count_work)))
         t = L.Tallies()
         se = np.zeros(n_px, np.uint32) if per_pixel else None
         bse = np.zeros(n_px, np.uint32) if per_pixel else None

@@ -288,8 +288,8 @@ def test_tiny_stack_waits_for_room_and_counts_drops():
         with pytest.raises(L.NblError) as e:
             drv.run_persistent(beam, stack_depth=64, overflow_depth=1)
         assert e.value.code == L.NBL_ERR_RUNTIME and "electrons stopped" in str(e.value)
-        t = drv.run_persistent(beam, stack_depth=64, overflow_depth=1, allow_drops=True)
-        ref = drv.run_persistent(beam)
+        t = drv.run_persistent(beam, stack_depth=64, overflow_depth=1, allow_drops=True, count_work=True)
+        ref = drv.run_persistent(beam, count_work=True)
     print(f"  tiny: pending {t['pending_retries']} stopped {t['stopped']} secondaries {t['secondaries']}; "
           f"default: pending {ref['pending_retries']} stopped {ref['stopped']}")
     assert t["primaries"] == ref["primaries"] == 20_000
