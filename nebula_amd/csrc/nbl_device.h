@@ -443,6 +443,11 @@ __device__ __forceinline__ float leaf_candidate(const tri_rec* __restrict__ recs
 {
     const float4* rp = reinterpret_cast<const float4*>(recs + k);
     const float4 a = rp[0], b = rp[1], c = rp[2];
+    // The record's three loads go out together: left to itself, the compiler
+    // issues the id's load alone and the others only after the id test,
+    // two round trips to L2 for every candidate.
+    asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(b.w), "v"(c.x),
+                 "v"(c.y), "v"(c.z));
     id = __float_as_uint(c.z);
     if (id == ignore_tri) return -1;
     const f3 r0{ a.x, a.y, a.z }, e1{ a.w, b.x, b.y }, e2{ b.z, b.w, c.x };
