@@ -583,14 +583,18 @@ NBL_HD float clampr(float i, float lo, float hi) { return i < lo ? lo : (i > hi 
 NBL_HD float minr(float a, float b) { return (b < a) ? b : a; }
 NBL_HD float maxr(float a, float b) { return (a < b) ? b : a; }
 
-__device__ __forceinline__ float tab1_get(const dtab1& t, float x)
+// table_1D::get in two parts, so that independent lookups can have their
+// loads in flight together: the fetch (index, the two values, the fraction)
+// and the interpolation.
+struct tab1_fetch { float v0, v1, f; };
+__device__ __forceinline__ tab1_fetch tab1_load(const dtab1& t, float x)
 {
     const float xi = (x - t.x_min) * t.x_step;
     const int lo = clampi((int)floorf(xi), 0, t.n - 2);
-    const float v0 = gptr(t.d)[lo], v1 = gptr(t.d)[lo + 1];
-    const float f = xi - (float)lo;
-    return (1 - f) * v0 + f * v1;
+    return { gptr(t.d)[lo], gptr(t.d)[lo + 1], xi - (float)lo };
 }
+__device__ __forceinline__ float tab1_finish(const tab1_fetch& q) { return (1 - q.f) * q.v0 + q.f * q.v1; }
+__device__ __forceinline__ float tab1_get(const dtab1& t, float x) { return tab1_finish(tab1_load(t, x)); }
 __device__ __forceinline__ float tab2_get(const dtab2& t, float x, float y)
 {
     const float xi = (x - t.x_min) * t.x_step;
@@ -792,15 +796,19 @@ template<typename R>
 __device__ __forceinline__ float sample_path(const dmat& m, float kin_energy, R& rng, uint8_t* type)
 {
     const float le = log_fast(kin_energy);
+    // both IMFP lookups' loads in flight before either is used (the
+    // compiler otherwise issues the second only after the first's draw)
+    const tab1_fetch q_el = tab1_load(m.el_imfp, le), q_in = tab1_load(m.in_imfp, le);
+    asm volatile("" ::"v"(q_el.v0), "v"(q_el.v1), "v"(q_in.v0), "v"(q_in.v1));
     float best = __builtin_inff();
     uint8_t ty = 0;
     {
-        const float imfp = exp_fast(tab1_get(m.el_imfp, le));
+        const float imfp = exp_fast(tab1_finish(q_el));
         const float path = rng_exponential(rng, rcp_fast(imfp));
         if (path < best) { best = path; ty = 2; }
     }
     {
-        const float imfp = exp_fast(tab1_get(m.in_imfp, le));
+        const float imfp = exp_fast(tab1_finish(q_in));
         const float path = rng_exponential(rng, rcp_fast(imfp));
         if (path < best) { best = path; ty = 1; }
     }
@@ -1068,14 +1076,22 @@ constexpr int OUT_ALIVE = 0, OUT_DETECTED = 1, OUT_TERMINATED = 2;
 template<typename R>
 __device__ __forceinline__ int intersect_execute(const dmat* mats, const tri_rec& tri, dparticle& p, int& mat, R& rng)
 {
+    const int t_in = (int)(int16_t)(tri.mats & 0xFFFFu), t_out = (int)(int16_t)(tri.mats >> 16);
+    // both sides' barriers are read first, so that their loads overlap the
+    // normal's arithmetic (the compiler would otherwise issue them after the
+    // special-material cases, which need the normal)
+    float bar_in = 0, bar_out = 0;
+    if (t_in >= 0) bar_in = mats[t_in].barrier;
+    if (t_out >= 0) bar_out = mats[t_out].barrier;
     const f3 nd = normalised(p.dir);
     const f3 e1{ tri.e1x, tri.e1y, tri.e1z }, e2{ tri.e2x, tri.e2y, tri.e2z };
     const f3 n = normalised(cross(e1, e2));
     const float cos_theta = dot(n, nd);
-    const int t_in = (int)(int16_t)(tri.mats & 0xFFFFu), t_out = (int)(int16_t)(tri.mats >> 16);
+    asm volatile("" ::"v"(bar_in), "v"(bar_out));
     int mat_in, mat_out;
-    if (cos_theta > 0) { mat_in = t_in; mat_out = t_out; }
-    else { mat_in = t_out; mat_out = t_in; }
+    float b_in, b_out;
+    if (cos_theta > 0) { mat_in = t_in; mat_out = t_out; b_in = bar_in; b_out = bar_out; }
+    else { mat_in = t_out; mat_out = t_in; b_in = bar_out; b_out = bar_in; }
     switch (mat_out) {
     case MAT_DETECTOR: return OUT_DETECTED;
     case MAT_DETECTOR_LT50: return p.e < 50 ? OUT_DETECTED : OUT_ALIVE;
@@ -1086,8 +1102,8 @@ __device__ __forceinline__ int intersect_execute(const dmat* mats, const tri_rec
     default: break;
     }
     float dU = 0;
-    if (mat_in >= 0) dU -= mats[mat_in].barrier;
-    if (mat_out >= 0) dU += mats[mat_out].barrier;
+    if (mat_in >= 0) dU -= b_in;
+    if (mat_out >= 0) dU += b_out;
     if (p.e * cos_theta * cos_theta + dU > 0) {
         const float s = sqrtf(1 + dU / (p.e * cos_theta * cos_theta));
         const float T = 4 * s / ((1 + s) * (1 + s));

@@ -44,8 +44,16 @@ template<bool COUNT, typename R>
 __device__ __forceinline__ bool init_sample(const dgeom& g, const dmat* __restrict__ mats, float min_energy,
     float max_energy, dparticle& p, int mat, R& rng, lane_counters& c, float& dist, uint8_t& type)
 {
-    return init_sample_in<COUNT>(g, mats[mat >= 0 ? mat : 0], mat >= 0, min_energy, max_energy, p, rng, c, dist,
-                                 type);
+    // The fields path sampling reads, loaded together before the energy
+    // test (the compiler would issue the tables' descriptors only after it).
+    const dmat& src = mats[mat >= 0 ? mat : 0];
+    dmat m;
+    m.barrier = src.barrier;
+    m.el_imfp = src.el_imfp;
+    m.in_imfp = src.in_imfp;
+    asm volatile("" ::"v"(m.barrier), "v"(m.el_imfp.d), "v"(m.el_imfp.n), "v"(m.el_imfp.x_min), "v"(m.el_imfp.x_step),
+                 "v"(m.in_imfp.d), "v"(m.in_imfp.n), "v"(m.in_imfp.x_min), "v"(m.in_imfp.x_step));
+    return init_sample_in<COUNT>(g, m, mat >= 0, min_energy, max_energy, p, rng, c, dist, type);
 }
 
 // The move after the walk: to the sampled event (EV_NONE, EV_INELASTIC or
