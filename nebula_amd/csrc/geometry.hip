@@ -115,6 +115,7 @@ int nbl_geometry_create(const nbl_triangle* tris, uint32_t n, nbl_geometry* out)
         NBL_HIP(hipDeviceSynchronize());
         dgeom& v = g->view;
         v.nodes = g->nodes.p; v.recs = g->recs.p; v.leaves = g->leaves.p; v.ropes = g->ropes.p; v.n_ropes = (uint32_t)std::max<size_t>(g->host.ropes.size(), 6); v.id_rec = g->id_rec.p;
+        v.owner_payload = g->host.recs.size() <= ((size_t)1 << OWNER_PAYLOAD_BITS) ? 1u : 0u;
         v.cx = g->host.center[0]; v.cy = g->host.center[1]; v.cz = g->host.center[2];
         v.hx = g->host.halfsize[0]; v.hy = g->host.halfsize[1]; v.hz = g->host.halfsize[2];
         v.mx = ldexpf(std::fabs(v.cx) + v.hx, -16);

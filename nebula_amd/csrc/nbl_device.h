@@ -113,6 +113,7 @@ struct dgeom {
     const leaf_info* leaves;   // per leaf, indexed by its child word
     const leaf_rope* ropes;    // 6 per leaf
     uint32_t n_ropes;
+    uint32_t owner_payload;    // 1: record indices fit 26 bits (the wavefront walk's owner search carries them)
     const uint32_t* id_rec;    // input triangle id -> index of one of its records
     float cx, cy, cz;          // root centre
     float hx, hy, hz;          // root half size

@@ -64,6 +64,24 @@ __device__ __forceinline__ uint32_t pooled_owner(uint32_t off, uint32_t count, u
     return wave_inclusive_max(mark);
 }
 
+// pooled_owner carrying a payload < 2^26 of the owner lane along: the
+// permute and the running maximum act on (lane << 26 | payload), whose
+// order is the lanes' order, so every lane receives its owner's payload
+// with its owner (key >> 26) and a round needs no further lane move.
+constexpr uint32_t OWNER_PAYLOAD_BITS = 26;
+__device__ __forceinline__ uint32_t pooled_owner_key(uint32_t off, uint32_t count, uint32_t base, uint32_t payload)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t key = (lane << OWNER_PAYLOAD_BITS) | payload;
+    const uint32_t slot = off - base < 64u ? off - base : 0u;
+    uint32_t mark = (uint32_t)__builtin_amdgcn_ds_permute((int)(slot * 4u), (int)key);
+    const uint64_t started = __ballot(count > 0 && off <= base);
+    const uint32_t open = 63u - (uint32_t)__builtin_clzll(started | 1ull);
+    const uint32_t open_key = (uint32_t)__builtin_amdgcn_readlane((int)key, (int)open);
+    if (lane == 0) mark = open_key;
+    return wave_inclusive_max(mark);
+}
+
 // A lane's walk between cells (the state octree::propagate keeps across
 // iterations of its cell loop, octree.inl:32-190).
 struct walk_state {
@@ -110,10 +128,21 @@ __device__ __forceinline__ bool walk_cell(bool run, const dgeom& g, walk_state& 
     const uint32_t off = wave_exclusive_sum(count, total);
     uint32_t best_id = NO_TRI;
     // candidates [off, off + count) of every lane, 64 per round
+    // a candidate's record is (first - off) + item of its owner: carried with
+    // the owner search where record indices fit its payload
+    const uint32_t rel = first - off;
     for (uint32_t base = 0; base < total; base += 64) {
         const uint32_t item = base + lane;
-        const uint32_t owner = pooled_owner(off, count, base);
-        const uint32_t k = __shfl(first - off, (int)owner) + item;
+        uint32_t owner, k;
+        if (g.owner_payload) {
+            constexpr uint32_t mask = (1u << OWNER_PAYLOAD_BITS) - 1;
+            const uint32_t key = pooled_owner_key(off, count, base, rel & mask);
+            owner = key >> OWNER_PAYLOAD_BITS;
+            k = (key + item) & mask;
+        } else {
+            owner = pooled_owner(off, count, base);
+            k = __shfl(rel, (int)owner) + item;
+        }
         const f3 op{ __shfl(w.pos.x, (int)owner), __shfl(w.pos.y, (int)owner), __shfl(w.pos.z, (int)owner) };
         const f3 od{ __shfl(dir.x, (int)owner), __shfl(dir.y, (int)owner), __shfl(dir.z, (int)owner) };
         const uint32_t oit = __shfl(ignore_tri, (int)owner);
