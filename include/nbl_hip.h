@@ -454,9 +454,9 @@ typedef struct {
                                   lookups_*, rounddowns, node_steps, tri_tests,
                                   boundary_events, secondaries, pending_retries;
                                   0 (default): they stay 0 and the launch runs about 4 %
-                                  faster.  Runs with a learned sampler or a deposit tally
-                                  always count.  primaries, detected_* and stopped are
-                                  always filled.                                          */
+                                  faster.  Runs with a deposit tally always count.
+                                  primaries, detected_* and stopped are always
+                                  filled.                                                 */
 } nbl_persistent_opts;
 
 /* Runs every primary of the batch to the end of its cascade in one

@@ -48,6 +48,12 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// The networks' parameters as global memory (address space 1): loads
+// through generic pointers would be flat loads, which also wait on the LDS
+// and scalar-load counter the network's lane permutes use.
+typedef __attribute__((address_space(1))) const bf16x4 gbf16x4;
+typedef __attribute__((address_space(1))) const bf16x8 gbf16x8;
+typedef __attribute__((address_space(1))) const float4 gfloat4;
 
 // Device view of one network.
 struct dmlp {
@@ -144,7 +150,7 @@ __device__ __forceinline__ float mlp_eval_blocks(const dmlp& n, int mat, float l
         if (g == 0) {
             x0 = s0; x1 = s1; x2 = s2; x3 = s3;
         } else if (g < 3) {
-            const float4 e = *reinterpret_cast<const float4*>(n.emb + (size_t)sm * MLP_EMB + 4 * (g - 1));
+            const float4 e = *(gfloat4*)(n.emb + (size_t)sm * MLP_EMB + 4 * (g - 1));
             x0 = e.x; x1 = e.y; x2 = e.z; x3 = e.w;
         }
         const bf16x4 bin = pack4(x0, x1, x2, x3);
@@ -152,10 +158,10 @@ __device__ __forceinline__ float mlp_eval_blocks(const dmlp& n, int mat, float l
         bf16x4 bh[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            const bf16x4* wm = n.w1 + m * 64;
+            gbf16x4* wm = (gbf16x4*)(n.w1 + m * 64);
             asm volatile("" : "+s"(wm));
             const int o = 16 * m + 4 * (int)g;
-            const float4 bb = *reinterpret_cast<const float4*>(n.b1 + o);
+            const float4 bb = *(gfloat4*)(n.b1 + o);
             f32x4 acc = { bb.x, bb.y, bb.z, bb.w };
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wm[lane], bin, acc, 0, 0, 0);
             bh[m] = relu_pack(acc);
@@ -172,10 +178,10 @@ __device__ __forceinline__ float mlp_eval_blocks(const dmlp& n, int mat, float l
                 // so its operand addresses are formed here (scalar base + lane
                 // offset) and not hoisted out of the loop as 64-bit vector
                 // addresses, which would hold many registers.
-                const bf16x8* wm = w + m * 2 * 64;
+                gbf16x8* wm = (gbf16x8*)(w + m * 2 * 64);
                 asm volatile("" : "+s"(wm));
                 const int o = 16 * m + 4 * (int)g;
-                const float4 bb = *reinterpret_cast<const float4*>(b + o);
+                const float4 bb = *(gfloat4*)(b + o);
                 f32x4 acc = { bb.x, bb.y, bb.z, bb.w };
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wm[lane], in0, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wm[64 + lane], in1, acc, 0, 0, 0);
@@ -185,7 +191,7 @@ __device__ __forceinline__ float mlp_eval_blocks(const dmlp& n, int mat, float l
         // output layer: row 0 of a product whose other rows are zero; the
         // value of sample j + 16 nb lands in lane j, element 0
         {
-            const bf16x8* wm = n.w4;
+            gbf16x8* wm = (gbf16x8*)n.w4;
             asm volatile("" : "+s"(wm));
             f32x4 acc = { 0, 0, 0, 0 };
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wm[lane], join8(bh[0], bh[1]), acc, 0, 0, 0);

@@ -203,7 +203,7 @@ __device__ void make_primary(const pers_params& P, uint64_t i, dparticle& p, uin
 // wave-uniform sums and two per-lane ones updated every loop iteration, and
 // cost about 4 % of the kernel time (more scalar registers than the loop
 // has, so they live in vector-register lanes), so runs that do not report
-// them skip them.
+// them skip them.  The deposit kernels always count.
 template<bool LEARNED, bool DEPOSIT, bool PENN, bool WORK>
 __global__ __launch_bounds__(LEARNED && DEPOSIT ? 768 : 1024) __attribute__((amdgpu_waves_per_eu(LEARNED ? 1 : 5)))
 void k_persistent(pers_params P)
@@ -498,7 +498,7 @@ struct launch_shape {
     int cus = 0;
     uint32_t block = 128, per_cu = 10, depth = 96, thresh = 4, ovf = 16384, ibatch = 8, iwait = 8;
     bool ovf_given = false, allow_drops = false, penn = false;
-    bool work = false;                  // keep the work counters (the learned and deposit kernels always do)
+    bool work = false;                  // keep the work counters (the deposit kernels always do)
     size_t lds = 0;
     size_t n_waves() const { return (size_t)cus * per_cu * (block / 64); }
     uint64_t n_threads() const { return (uint64_t)cus * per_cu * block; }
@@ -524,7 +524,7 @@ launch_shape resolve_shape(nbl_driver d, const nbl_persistent_opts* opts, bool d
     if (opts && opts->inelastic_batch) L.ibatch = opts->inelastic_batch;
     if (opts && opts->inelastic_wait) L.iwait = opts->inelastic_wait;
     L.allow_drops = opts && opts->allow_drops;
-    L.work = (opts && opts->count_work) || d->sampler || deposit;
+    L.work = (opts && opts->count_work) || deposit;
     require(L.ibatch <= 64, "nbl_driver_run_persistent: inelastic_batch must be 1..64");
     require(L.iwait <= 1024, "nbl_driver_run_persistent: inelastic_wait must be <= 1024");
     require(L.ovf <= (1u << 20), "nbl_driver_run_persistent: overflow_depth must be <= 2^20");
@@ -571,7 +571,8 @@ float launch_transport(nbl_driver d, const launch_shape& L, const pers_params& P
     const dim3 grid((unsigned)(L.cus * L.per_cu)), threads_per_block(L.block);
     if (d->sampler) {
         if (deposit) hipLaunchKernelGGL((k_persistent<true, true, false, true>), grid, threads_per_block, L.lds, s, P);
-        else hipLaunchKernelGGL((k_persistent<true, false, false, true>), grid, threads_per_block, L.lds, s, P);
+        else if (L.work) hipLaunchKernelGGL((k_persistent<true, false, false, true>), grid, threads_per_block, L.lds, s, P);
+        else hipLaunchKernelGGL((k_persistent<true, false, false, false>), grid, threads_per_block, L.lds, s, P);
     } else if (L.penn) {
         if (deposit) hipLaunchKernelGGL((k_persistent<false, true, true, true>), grid, threads_per_block, L.lds, s, P);
         else if (L.work) hipLaunchKernelGGL((k_persistent<false, false, true, true>), grid, threads_per_block, L.lds, s, P);
