@@ -53,7 +53,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // and scalar-load counter the network's lane permutes use.
 typedef __attribute__((address_space(1))) const bf16x4 gbf16x4;
 typedef __attribute__((address_space(1))) const bf16x8 gbf16x8;
-typedef __attribute__((address_space(1))) const float4 gfloat4;
+typedef __attribute__((address_space(1))) const f32x4 gf32x4;
 
 // Device view of one network.
 struct dmlp {
@@ -150,7 +150,7 @@ __device__ __forceinline__ float mlp_eval_blocks(const dmlp& n, int mat, float l
         if (g == 0) {
             x0 = s0; x1 = s1; x2 = s2; x3 = s3;
         } else if (g < 3) {
-            const float4 e = *(gfloat4*)(n.emb + (size_t)sm * MLP_EMB + 4 * (g - 1));
+            const f32x4 e = *(gf32x4*)(n.emb + (size_t)sm * MLP_EMB + 4 * (g - 1));
             x0 = e.x; x1 = e.y; x2 = e.z; x3 = e.w;
         }
         const bf16x4 bin = pack4(x0, x1, x2, x3);
@@ -161,7 +161,7 @@ __device__ __forceinline__ float mlp_eval_blocks(const dmlp& n, int mat, float l
             gbf16x4* wm = (gbf16x4*)(n.w1 + m * 64);
             asm volatile("" : "+s"(wm));
             const int o = 16 * m + 4 * (int)g;
-            const float4 bb = *(gfloat4*)(n.b1 + o);
+            const f32x4 bb = *(gf32x4*)(n.b1 + o);
             f32x4 acc = { bb.x, bb.y, bb.z, bb.w };
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wm[lane], bin, acc, 0, 0, 0);
             bh[m] = relu_pack(acc);
@@ -181,7 +181,7 @@ __device__ __forceinline__ float mlp_eval_blocks(const dmlp& n, int mat, float l
                 gbf16x8* wm = (gbf16x8*)(w + m * 2 * 64);
                 asm volatile("" : "+s"(wm));
                 const int o = 16 * m + 4 * (int)g;
-                const float4 bb = *(gfloat4*)(b + o);
+                const f32x4 bb = *(gf32x4*)(b + o);
                 f32x4 acc = { bb.x, bb.y, bb.z, bb.w };
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wm[lane], in0, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wm[64 + lane], in1, acc, 0, 0, 0);
