@@ -296,3 +296,25 @@ def test_tiny_stack_waits_for_room_and_counts_drops():
     assert t["pending_retries"] > 0 and ref["stopped"] == 0
     assert 0 < t["secondaries"] and t["stopped"] < t["secondaries"]
     assert t["detected_bse"] > 0 and t["detected_se"] > 0
+
+
+def test_work_counters_on_request():
+    """nbl_persistent_opts.count_work: the default launch leaves the work
+    counters at 0 and still counts primaries, detections and stopped
+    electrons; with count_work every work counter is filled, consistently
+    (a step per event, two free-path lookups per step in a material)."""
+    print("[gpu] work counters on request", flush=True)
+    g = na.Geometry(scenes.cube_slab()); m = na.Materials([synth.SI])
+    beam = raster_scan(np.zeros(1), np.zeros(1), 5.0, 1000.0, 50_000, 0.0, seed=3)
+    work = ("events", "lookups_1d", "lookups_2d", "rounddowns", "node_steps", "tri_tests", "boundary_events",
+            "secondaries")
+    with Driver(g, m, capacity=1024, seed=8) as drv:
+        a = drv.run_persistent(beam)
+        b = drv.run_persistent(beam, count_work=True)
+    for t in (a, b):
+        assert t["primaries"] == 50_000 and t["stopped"] == 0
+        assert t["detected_se"] > 0 and t["detected_bse"] > 0
+    for k in work:
+        assert a[k] == 0 and b[k] > 0, (k, a[k], b[k])
+    assert b["events"] > b["boundary_events"] and b["lookups_1d"] <= 2 * b["events"]
+    assert b["lookups_2d"] >= b["rounddowns"]
